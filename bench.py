@@ -1,0 +1,190 @@
+"""Throughput benchmark of the per-step PBF solve on one MI355X.
+
+Default workload: the 1,000,000-particle dam break (100^3 jittered lattice at
+spacing h/2, h = 0.1, box 12 x 6 x 6, PAPER physics, 4 constraint
+iterations, FP32).  A step is one pass of the hot path: predict, cell sort,
+4 x (lambda, delta), XSPH + vorticity, position update, launched as one
+captured hipGraph.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--config 1] [--physics paper]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "position-based-fluids_amd"))
+
+import numpy as np  # noqa: E402
+
+import pbf  # noqa: E402
+import scenarios  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) HBM bytes per particle per launch of each kernel
+# class, PAPER physics, FP32; "delta" is the mean over the I launches (the
+# last one also reads rho and x and writes v).
+def algorithmic_bytes(cls: str, iterations: int, physics: int) -> float:
+    last_extra = 36.0 / iterations
+    table = {
+        "predict_count": 56.0,
+        "scatter": 24.0,
+        "reorder": 88.0,
+        "lambda": 24.0,
+        "delta": 36.0 + last_extra,
+        "xsph": 64.0 if physics == pbf.PHYSICS_PAPER else 84.0,
+        "vorticity": 88.0,
+    }
+    return table.get(cls, 0.0)
+
+
+def step_bytes(n: int, cells: int, iterations: int, physics: int) -> float:
+    """Compulsory bytes per step: 324 + 60 I + 16 C/N per particle (PAPER),
+    244 + 60 I + 16 C/N (REF)."""
+    base = 324.0 if physics == pbf.PHYSICS_PAPER else 244.0
+    return n * (base + 60.0 * iterations) + 16.0 * cells
+
+
+def load_traffic(workload: str):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline(params, parts, steps: int, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pbf_oracle as orc
+
+    orc.set_threads(threads)
+    pos, vel = orc.state_from_particles(parts)
+    t0 = time.perf_counter()
+    orc.run_f32(params, pos, vel, steps)
+    dt = time.perf_counter() - t0
+    return parts.shape[0] * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 4])
+    ap.add_argument("--physics", choices=["paper", "ref"], default="paper")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    physics = pbf.PHYSICS_PAPER if args.physics == "paper" else pbf.PHYSICS_REF
+    sc = scenarios.config(args.config, physics=physics)
+    params, parts = sc.params[0], sc.particles[0]
+    if args.block:
+        params.block_size = args.block
+    n = sc.n
+    iters = params.iterations
+    print(f"[bench] {sc.name}: {n} particles, {iters} iterations, "
+          f"{args.physics} physics, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+
+    sim = pbf.Simulation(params, parts)
+    cells = sim.num_cells()
+    sim.step(args.warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    ms = sim.time_steps(args.steps)          # HIP events around the graph launches
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    value = n * args.steps / (ms / 1e3)
+    print(f"[bench] timed {args.steps} steps: {ms:.3f} ms (wall {wall*1e3:.1f} ms)",
+          file=sys.stderr, flush=True)
+
+    diag = sim.diagnostics()
+    prof = sim.profile(args.profile_steps)
+    cfg = sim.config()
+    sim.close()
+
+    total_prof = sum(v[0] for v in prof.values())
+    dom = max(prof, key=lambda k: prof[k][0])
+    dom_ms, dom_launches = prof[dom]
+    avg_ms = dom_ms / max(dom_launches, 1)
+    dom_bytes = algorithmic_bytes(dom, iters, physics) * n
+    achieved = dom_bytes / (avg_ms / 1e3) / 1e9
+    workload = f"c{args.config}"
+    traffic = load_traffic(workload)
+
+    b_step = step_bytes(n, cells, iters, physics)
+    step_gbs = b_step * args.steps / (ms / 1e3) / 1e9
+
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{sc.name}: dam break, {n} particles, "
+                         f"{iters} iterations, {args.physics} physics, FP32"),
+            "particles": n,
+            "cells": cells,
+            "iterations": iters,
+            "physics": args.physics,
+            "block": cfg[0],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": (traffic or {}).get("bytes_per_launch"),
+            "avg_launch_ms": avg_ms,
+            "algorithmic_bytes_per_launch": dom_bytes,
+        },
+        "step_roofline": {
+            "bytes_per_step": b_step,
+            "achieved_GBps": step_gbs,
+            "frac": step_gbs / HBM_PEAK_GBS,
+        },
+        "kernel_ms_per_step": {k: v[0] / args.profile_steps for k, v in prof.items()},
+        "profile_overhead": total_prof / args.profile_steps / (ms / args.steps),
+        "accuracy": {
+            "max_compression": diag["max_compression"],
+            "mean_abs_density_err": diag["mean_abs_density_err"],
+        },
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cps, secs = cpu_baseline(params, parts, args.cpu_steps, threads)
+        out["cpu_baseline"] = {
+            "value": cps,
+            "unit": "particle-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": (f"oracle FP32 restatement, same {sc.name} input and "
+                       f"{args.physics} physics, {args.cpu_steps} step(s), "
+                       f"{secs:.1f} s on {threads} threads"),
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,220 @@
+/*
+ * pbf.h - C ABI of the MI355X position-based-fluids solver.
+ *
+ * This is the drop-in boundary for the reference's per-step path.  The
+ * reference exposes that path as the C++ class `Simulation`
+ * (src/Simulation.hpp:30-197) built on an OpenCL kernel map
+ * (src/ocl/clsetup.cpp:344-358, src/main.cpp:170-175).  Every entry point
+ * below names the reference member it replaces.  No exceptions cross this
+ * boundary: every call returns an int status (PBF_OK == 0) and the text of
+ * the last failure on the calling thread is available from pbf_last_error().
+ *
+ * Threading: one HIP stream per handle; a handle is not thread-safe, callers
+ * serialise calls on the same handle.  Distinct handles are independent.
+ *
+ * Array conventions: particle arrays are float4 records (4 floats per
+ * particle) in ORIGINAL particle order (the order of the input particle
+ * array), exactly as Simulation::dumpData returns them
+ * (src/Simulation.cpp:578-591): pos = (x, y, z, |v|), vel = (vx, vy, vz, m).
+ */
+#ifndef PBF_H
+#define PBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PBF_OK 0
+#define PBF_ERR_INVALID 1   /* bad argument (null, size, range)              */
+#define PBF_ERR_HIP 2       /* HIP runtime failure (no device, OOM, launch)  */
+#define PBF_ERR_STATE 3     /* call not valid for this handle                */
+#define PBF_ERR_NUMERIC 4   /* device error word set (NaN position)          */
+
+/* ---- physics modes ----------------------------------------------------- */
+/* PBF_PHYSICS_REF: the equations of the reference kernels
+ *   (src/kernels/compute_scaling.cl:12-140, compute_delta.cl:13-152,
+ *    apply_vorticity_and_viscosity.cl:12-104), with the kernel constants
+ *   rounded to 6 significant digits as src/main.cpp:149-168 formats them.
+ * PBF_PHYSICS_PAPER: Macklin & Mueller 2013 (eqs. 1, 2, 8, 11, 13-17):
+ *   relaxation epsilon, s_corr, XSPH and vorticity confinement, exact
+ *   constants. */
+#define PBF_PHYSICS_REF 0
+#define PBF_PHYSICS_PAPER 1
+
+/* ---- precision --------------------------------------------------------- */
+/* PBF_PRECISION_MIXED: the first (iterations - fp32_iterations) constraint
+ * iterations gather FP16 cell-relative positions (kernels evaluated and
+ * accumulated in FP32); the remaining iterations are full FP32. */
+#define PBF_PRECISION_FP32 0
+#define PBF_PRECISION_MIXED 1
+
+/* Simulation parameters.  The first block maps 1:1 onto the reference's
+ * ConfigParameters (src/Parameters.hpp:12-31) as read from a .par file
+ * (src/io/ConfigReader.cpp:19-94); the file-name and output fields stay in
+ * the C++ facade.  The second block holds what the reference hard-codes in
+ * its kernels and host (epsilon compute_scaling.cl:16, gravity
+ * predict_positions.cl:8, XSPH c apply_vorticity_and_viscosity.cl:103,
+ * iteration count Simulation.cpp:503).  pbf_params_init fills defaults. */
+typedef struct pbf_params {
+  float time_step_length; /* timestep_length                                */
+  float time_end;         /* time_end (used by the Runner facade only)      */
+  float x_min, x_max, y_min, y_max, z_min, z_max; /* box                    */
+  float x_n, y_n, z_n;    /* cell counts per axis (floats, as reference)    */
+  float rest_density;     /* restdensity                                    */
+
+  int32_t physics;        /* PBF_PHYSICS_*                                  */
+  int32_t iterations;     /* constraint iterations per step (reference: 4)  */
+  float relaxation;       /* epsilon in the lambda denominator              */
+  float scorr_k;          /* s_corr k          (paper only)                 */
+  float scorr_n;          /* s_corr exponent n (paper only)                 */
+  float scorr_dq;         /* s_corr delta-q as a fraction of h (paper only) */
+  float xsph_c;           /* XSPH viscosity c                               */
+  float vorticity_eps;    /* vorticity confinement epsilon (paper only)     */
+  float gravity_y;        /* gravity along y (reference: -9.81)             */
+  int32_t precision;      /* PBF_PRECISION_*                                */
+  int32_t fp32_iterations;/* mixed precision: trailing full-FP32 iterations */
+  float sphere[4];        /* obstacle sphere (cx, cy, cz, r); r <= 0: none  */
+  int32_t block_size;     /* gather work-group size; 0 = autotuned          */
+  int32_t reserved[7];
+} pbf_params;
+
+/* One input particle, layout of the reference's Particle
+ * (src/Particle.hpp:9-14): mass, position, velocity, unused acceleration. */
+typedef struct pbf_particle {
+  float m;
+  float x[3];
+  float v[3];
+  float a[3];
+} pbf_particle;
+
+typedef struct pbf_sim pbf_sim;
+
+/* Fill *p with the defaults of a physics mode (box and cell counts are left
+ * zero).  REF: epsilon 1e4, c 0.01, g -9.81, 4 iterations, no s_corr, no
+ * vorticity.  PAPER: epsilon 600, s_corr k 0.001 n 4 dq 0.2, c 0.01,
+ * vorticity 1e-4, g -9.81, 4 iterations.  No reference counterpart. */
+int pbf_params_init(pbf_params *p, int32_t physics);
+
+/* Replaces Simulation::Simulation + init() + initCells()
+ * (src/Simulation.hpp:41-46, src/Simulation.cpp:34-88,104-230): validates
+ * parameters, allocates device state on the current HIP device, uploads the
+ * particles.  n may be 0 (every later call is then a no-op). */
+int pbf_create(const pbf_params *params, const pbf_particle *particles,
+               uint32_t n, pbf_sim **out);
+
+/* Many independent scenes in one handle; every kernel of a step covers all
+ * scenes in one launch.  scene s has counts[s] particles at particles[s].
+ * No reference counterpart (the reference simulates one scene). */
+int pbf_batch_create(uint32_t nscenes, const pbf_params *params,
+                     const pbf_particle *const *particles,
+                     const uint32_t *counts, pbf_sim **out);
+
+/* Replaces Simulation::step (src/Simulation.cpp:455-576), nsteps times.
+ * Asynchronous with respect to the host; pbf_synchronize / pbf_dump wait. */
+int pbf_step(pbf_sim *sim, uint32_t nsteps);
+
+/* Block the host until all work queued on the handle's stream is done.
+ * Replaces the mQueue.finish() at src/Simulation.cpp:573. */
+int pbf_synchronize(pbf_sim *sim);
+
+/* Replaces Simulation::dumpData (src/Simulation.cpp:578-591): fills
+ * caller-owned float4[n] arrays (either may be NULL) in original order. */
+int pbf_dump(pbf_sim *sim, float *pos4, float *vel4);
+
+/* Batched handle: dump one scene. */
+int pbf_batch_dump(pbf_sim *sim, uint32_t scene, float *pos4, float *vel4);
+
+/* Replaces Simulation::setWaveGenerator (src/Simulation.hpp:105-108):
+ * moving x-min wall offset used by the collision clamp
+ * (src/kernels/compute_delta.cl:130-131).  Applies to every scene. */
+int pbf_set_wave_generator(pbf_sim *sim, float value);
+
+/* Replaces getSizesMin/getSizesMax (src/Simulation.hpp:93-101). */
+int pbf_get_box(const pbf_sim *sim, float min4[4], float max4[4]);
+
+/* Replaces getNumberParticles (src/Simulation.hpp:61-63); batched handles
+ * return the total over scenes. */
+uint32_t pbf_num_particles(const pbf_sim *sim);
+uint32_t pbf_num_scenes(const pbf_sim *sim);
+
+/* Replaces ~Simulation (src/Simulation.cpp:90-101).  NULL is accepted. */
+int pbf_destroy(pbf_sim *sim);
+
+/* Thread-local text of the last failure ("" if none).  No reference
+ * counterpart (the reference throws cl::Error / std::runtime_error,
+ * src/main.cpp:180-186). */
+const char *pbf_last_error(void);
+
+/* ---- scenario loading ---------------------------------------------------- */
+
+/* Replaces ConfigReader::read (src/io/ConfigReader.cpp:19-94): reads a .par
+ * file ("key value" lines; the reference keys plus the physics keys
+ * physics ref|paper, iterations, relaxation, scorr_k, scorr_n, scorr_dq,
+ * xsph_c, vorticity_eps, gravity_y, precision fp32|mixed, fp32_iterations,
+ * sphere cx cy cz r, block_size).  Missing keys keep the REF defaults of
+ * pbf_params_init (zero for the box).  part_input_file is copied into
+ * part_file (NUL-terminated, at most cap bytes) when part_file != NULL. */
+int pbf_read_par(const char *path, pbf_params *out, char *part_file,
+                 size_t cap);
+
+/* Replaces PartReader::read (src/io/PartReader.cpp:21-97): reads a .in
+ * particle file; *n receives the number of particles read and at most cap
+ * of them are copied to out (out may be NULL to query the count). */
+int pbf_read_particles(const char *path, pbf_particle *out, uint32_t cap,
+                       uint32_t *n);
+
+/* ---- parity hooks and measurement (no reference counterpart) ----------- */
+
+/* Run the neighbour-search stage of a step on the current state without
+ * advancing it: predict (predict_positions.cl:5-9), cell assignment
+ * (update_cells.cl:9-15), the sort, and one density pass.  Outputs, each
+ * optional (NULL to skip), single-scene handles only:
+ *   cell_of[n]      cell id of each particle (original order)
+ *   sorted_ids[n]   original particle id at each sorted slot
+ *   cell_start[C+1] first sorted slot of each cell, cell_start[C] = n
+ *   nbr_count[n]    neighbours j != i with 0 < r^2 < h^2 in the 27-cell
+ *                   stencil (original order)
+ *   rho[n], lambda[n]  density and lambda of the first constraint pass
+ *   pred4[n]        predicted positions x* (float4, original order)      */
+int pbf_probe(pbf_sim *sim, int32_t *cell_of, int32_t *sorted_ids,
+              int32_t *cell_start, int32_t *nbr_count, float *rho,
+              float *lambda, float *pred4);
+
+/* Number of cells C of a single-scene handle (length of cell_start is C+1). */
+uint32_t pbf_num_cells(const pbf_sim *sim);
+
+/* FP64 reductions on the device over the current state (all scenes):
+ * out[0] = max |rho_i/rho0 - 1| over particles with at least one neighbour,
+ * out[1] = mean |rho_i/rho0 - 1| over the same particles,
+ * out[2..4] = total momentum sum m v, out[5] = kinetic energy sum m v^2/2,
+ * out[6] = number of particles counted in out[0..1].
+ * The density is evaluated at the current positions. */
+int pbf_diagnostics(pbf_sim *sim, double out[8]);
+
+/* Run nsteps steps bracketing every kernel launch with HIP events on the
+ * handle's stream; writes per-kernel-class totals (milliseconds, summed
+ * over the nsteps steps) into ms[k] and launch counts into launches[k] for
+ * k < *nclass (in: capacity, out: number of classes), names[k] -> static
+ * strings.  The state advances as with pbf_step. */
+int pbf_profile_steps(pbf_sim *sim, uint32_t nsteps, double *ms,
+                      uint32_t *launches, const char **names,
+                      uint32_t *nclass);
+
+/* Time nsteps graph launches with HIP events on the handle's stream
+ * (after the caller's warm-up); returns elapsed milliseconds in *ms. */
+int pbf_time_steps(pbf_sim *sim, uint32_t nsteps, double *ms);
+
+/* Launch configuration chosen for this handle (after autotuning):
+ * cfg[0] = gather block size, cfg[1] = particles per gather work-item
+ * (1), cfg[2] = sort digit width in bits (0 = one-pass counting sort). */
+int pbf_get_config(const pbf_sim *sim, int32_t cfg[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBF_H */

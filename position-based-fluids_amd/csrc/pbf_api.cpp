@@ -1,0 +1,717 @@
+// pbf_api.cpp - host runtime behind the C ABI of include/pbf.h.
+//
+// Replaces the reference's Simulation + OpenCL plumbing
+// (src/Simulation.cpp, src/ocl/clsetup.cpp): owns the device state of one
+// or many scenes, sequences a step as one captured hipGraph on the handle's
+// stream, and converts every failure into a status code plus a
+// thread-local message (no exception crosses the ABI).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pbf.h"
+#include "pbf_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define PBF_HIP(call)                                                     \
+  do {                                                                    \
+    const hipError_t e_ = (call);                                         \
+    if (e_ != hipSuccess)                                                 \
+      throw ApiError(PBF_ERR_HIP, std::string(#call) + ": " +             \
+                                      hipGetErrorString(e_));             \
+  } while (0)
+
+[[noreturn]] void invalid(const std::string &m) {
+  throw ApiError(PBF_ERR_INVALID, m);
+}
+
+template <typename F>
+int guard(F &&f) {
+  try {
+    f();
+    g_error.clear();
+    return PBF_OK;
+  } catch (const ApiError &e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_error = "host allocation failed";
+    return PBF_ERR_HIP;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return PBF_ERR_HIP;
+  }
+}
+
+// 6-significant-digit rendering of a kernel constant: src/main.cpp:149-168
+// streams each value with std::showpoint at the default precision into a
+// -D flag, i.e. printf "%#g".
+float fmt6(double v) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%#g", v);
+  return std::strtof(buf, nullptr);
+}
+
+bool is_int_valued(float v) { return std::floor(v) == v; }
+
+void validate(const pbf_params &p) {
+  if (!(p.x_n >= 1 && p.y_n >= 1 && p.z_n >= 1) || !is_int_valued(p.x_n) ||
+      !is_int_valued(p.y_n) || !is_int_valued(p.z_n))
+    invalid("cell counts x_n, y_n, z_n must be integers >= 1");
+  if (!(p.x_max > p.x_min && p.y_max > p.y_min && p.z_max > p.z_min))
+    invalid("box must have max > min on every axis");
+  if ((double)p.x_n * p.y_n * p.z_n > 1.0e9)
+    invalid("too many cells (limit 1e9)");
+  if (!(p.time_step_length > 0.0f)) invalid("timestep_length must be > 0");
+  if (!(p.rest_density > 0.0f)) invalid("restdensity must be > 0");
+  if (p.physics != PBF_PHYSICS_REF && p.physics != PBF_PHYSICS_PAPER)
+    invalid("physics must be PBF_PHYSICS_REF or PBF_PHYSICS_PAPER");
+  if (p.iterations < 1 || p.iterations > 64)
+    invalid("iterations must be in [1, 64]");
+  if (p.precision != PBF_PRECISION_FP32 && p.precision != PBF_PRECISION_MIXED)
+    invalid("precision must be PBF_PRECISION_FP32 or PBF_PRECISION_MIXED");
+  if (p.precision == PBF_PRECISION_MIXED &&
+      (p.fp32_iterations < 1 || p.fp32_iterations > p.iterations))
+    invalid("mixed precision needs 1 <= fp32_iterations <= iterations");
+  if (p.precision == PBF_PRECISION_MIXED && p.physics != PBF_PHYSICS_PAPER)
+    invalid("mixed precision is available for PAPER physics only");
+  if (p.precision == PBF_PRECISION_MIXED)
+    invalid("mixed precision is not available in this build");
+  if (!(p.scorr_n >= 0 && p.scorr_n <= 16) || !is_int_valued(p.scorr_n))
+    invalid("scorr_n must be an integer in [0, 16]");
+  if (p.block_size != 0 && p.block_size != 64 && p.block_size != 128 &&
+      p.block_size != 256)
+    invalid("block_size must be 0, 64, 128 or 256");
+}
+
+pbf::Scene make_scene(const pbf_params &p) {
+  pbf::Scene s{};
+  const bool ref = p.physics == PBF_PHYSICS_REF;
+  const float bmin[3] = {p.x_min, p.y_min, p.z_min};
+  const float bmax[3] = {p.x_max, p.y_max, p.z_max};
+  const float n[3] = {p.x_n, p.y_n, p.z_n};
+  const float hf = (p.x_max - p.x_min) / p.x_n;  // main.cpp:164
+  for (int a = 0; a < 3; ++a) {
+    const float cl = (bmax[a] - bmin[a]) / n[a];  // main.cpp:159-161
+    if (ref) {
+      s.bmin[a] = fmt6(bmin[a]);
+      s.bmax[a] = fmt6(bmax[a]);
+      s.cell_len[a] = fmt6(cl);
+    } else {
+      s.bmin[a] = bmin[a];
+      s.bmax[a] = bmax[a];
+      s.cell_len[a] = (float)(((double)bmax[a] - (double)bmin[a]) / n[a]);
+    }
+    s.inv_cell[a] = 1.0f / s.cell_len[a];
+    s.nc[a] = (int)n[a];
+  }
+  s.ncells = s.nc[0] * s.nc[1] * s.nc[2];
+  const double pi = 3.14159265358979323846;
+  double h;
+  if (ref) {
+    h = (double)hf;
+    s.h = fmt6(hf);
+    s.h2 = fmt6(std::pow((double)hf, 2));
+    s.poly6 = fmt6(315.0 / (64.0 * pi * std::pow((double)hf, 9)));
+    s.spiky = fmt6(45.0 / (pi * std::pow((double)hf, 6)));
+    s.dt = fmt6(p.time_step_length);
+    s.rho0 = fmt6(p.rest_density);
+  } else {
+    h = ((double)p.x_max - (double)p.x_min) / (double)p.x_n;
+    s.h = (float)h;
+    s.h2 = (float)(h * h);
+    s.poly6 = (float)(315.0 / (64.0 * pi * std::pow(h, 9)));
+    s.spiky = (float)(45.0 / (pi * std::pow(h, 6)));
+    s.dt = p.time_step_length;
+    s.rho0 = p.rest_density;
+  }
+  s.inv_rho0 = 1.0f / s.rho0;
+  s.eps = p.relaxation;
+  s.c_xsph = p.xsph_c;
+  s.g_y = p.gravity_y;
+  s.scorr_k = p.scorr_k;
+  s.scorr_n = (int)p.scorr_n;
+  {
+    const double hh = (double)s.h, dq = (double)p.scorr_dq * hh;
+    const double t = hh * hh - dq * dq;
+    s.scorr_cq = (t > 0.0) ? (float)(1.0 / (t * t * t)) : 0.0f;
+  }
+  s.vort_eps = p.vorticity_eps;
+  s.wave = 0.0f;
+  for (int a = 0; a < 4; ++a) s.sphere[a] = p.sphere[a];
+  s.half_scale = (float)(1.0 / h);
+  return s;
+}
+
+struct EventTimer final : pbf::KernelTimer {
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t take() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      PBF_HIP(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void begin(int cls, hipStream_t st) override {
+    Rec r{cls, take(), nullptr};
+    PBF_HIP(hipEventRecord(r.a, st));
+    recs.push_back(r);
+  }
+  void end(int, hipStream_t st) override {
+    recs.back().b = take();
+    PBF_HIP(hipEventRecord(recs.back().b, st));
+  }
+  ~EventTimer() override {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+namespace pbf {
+// Error text of the scenario readers (io_capi.cpp).
+void set_error_text(const char *text) { g_error = text ? text : ""; }
+}  // namespace pbf
+
+struct pbf_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<pbf_params> params;
+  std::vector<pbf::Scene> scenes;
+  pbf::Scene *scenes_pinned = nullptr;
+  hipEvent_t upload_done = nullptr;
+  bool upload_pending = false;
+  bool dirty = true;
+  std::vector<int> block_scene;
+  pbf::StepShape shape{};
+  int total = 0;  // particles over scenes
+  pbf::DevBuffers d{};
+  std::vector<void *> allocs;
+  hipGraphExec_t graph = nullptr;
+  double *diag_out = nullptr;
+  float wave = 0.0f;
+
+  template <typename T>
+  T *alloc(size_t count) {
+    void *p = nullptr;
+    PBF_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    allocs.push_back(p);
+    return static_cast<T *>(p);
+  }
+
+  ~pbf_sim() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (void *p : allocs) (void)hipFree(p);
+    if (scenes_pinned) (void)hipHostFree(scenes_pinned);
+    if (upload_done) (void)hipEventDestroy(upload_done);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // Copy the scene table to the device if it changed (wave generator).
+  void upload_scenes() {
+    if (!dirty) return;
+    if (upload_pending) PBF_HIP(hipEventSynchronize(upload_done));
+    std::memcpy(scenes_pinned, scenes.data(),
+                scenes.size() * sizeof(pbf::Scene));
+    PBF_HIP(hipMemcpyAsync(d.scenes, scenes_pinned,
+                           scenes.size() * sizeof(pbf::Scene),
+                           hipMemcpyHostToDevice, stream));
+    PBF_HIP(hipEventRecord(upload_done, stream));
+    upload_pending = true;
+    dirty = false;
+  }
+
+  void check_err() {
+    int e = 0;
+    PBF_HIP(hipMemcpyAsync(&e, d.err, sizeof(int), hipMemcpyDeviceToHost,
+                           stream));
+    PBF_HIP(hipStreamSynchronize(stream));
+    if (e)
+      throw ApiError(PBF_ERR_NUMERIC,
+                     "non-finite particle position or velocity");
+  }
+};
+
+namespace {
+
+void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
+           const pbf_particle *const *parts, const uint32_t *counts) {
+  if (nscenes == 0) invalid("nscenes must be >= 1");
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    validate(params[s]);
+    if (counts[s] > 0 && !parts[s]) invalid("particles is NULL");
+    if (params[s].physics != params[0].physics ||
+        params[s].iterations != params[0].iterations ||
+        params[s].precision != params[0].precision ||
+        params[s].fp32_iterations != params[0].fp32_iterations)
+      invalid("all scenes of a batch need the same physics, iterations and "
+              "precision");
+  }
+  PBF_HIP(hipGetDevice(&S.device));
+  PBF_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+  PBF_HIP(hipEventCreateWithFlags(&S.upload_done, hipEventDisableTiming));
+  S.params.assign(params, params + nscenes);
+  long long slot = 0, cells = 0, ids = 0;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    pbf::Scene sc = make_scene(params[s]);
+    sc.n = (int)counts[s];
+    sc.slot_off = (int)slot;
+    sc.id_base = (int)ids;
+    sc.shift = (int)(slot - ids);
+    sc.cell_base = (int)cells;
+    const long long padded =
+        ((long long)counts[s] + pbf::kSlotAlign - 1) / pbf::kSlotAlign *
+        pbf::kSlotAlign;
+    for (long long b = 0; b < padded / pbf::kSlotAlign; ++b)
+      S.block_scene.push_back((int)s);
+    slot += padded;
+    ids += counts[s];
+    cells += sc.ncells;
+    if (slot > (1LL << 30) || cells > (1LL << 30))
+      invalid("problem too large (slots or cells beyond 2^30)");
+    S.scenes.push_back(sc);
+  }
+  S.total = (int)ids;
+  const pbf_params &p0 = params[0];
+  pbf::StepShape &sh = S.shape;
+  sh.slots = (int)slot;
+  sh.cells = (int)cells;
+  sh.scan_blocks = (int)((cells + pbf::kScanTile - 1) / pbf::kScanTile);
+  sh.physics = p0.physics == PBF_PHYSICS_REF ? pbf::kRef : pbf::kPaper;
+  sh.iterations = p0.iterations;
+  sh.fp16_iterations = p0.precision == PBF_PRECISION_MIXED
+                           ? p0.iterations - p0.fp32_iterations
+                           : 0;
+  bool vort = false;
+  for (uint32_t s = 0; s < nscenes; ++s)
+    vort = vort || params[s].vorticity_eps != 0.0f;
+  sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
+  sh.block = p0.block_size ? p0.block_size : 256;
+
+  const size_t P = (size_t)sh.slots, M = (size_t)sh.cells;
+  pbf::DevBuffers &d = S.d;
+  d.pos = S.alloc<float4>(P);
+  d.vel = S.alloc<float4>(P);
+  d.pid = S.alloc<int>(P);
+  d.pred_u = S.alloc<float4>(P);
+  d.key = S.alloc<int>(P);
+  d.rank = S.alloc<int>(P);
+  d.count = S.alloc<int>(M);
+  d.cell_start = S.alloc<int>(M + 1);
+  d.partial = S.alloc<int>((size_t)sh.scan_blocks + 1);
+  d.perm = S.alloc<int>(P);
+  d.ppid = S.alloc<int>(P);
+  d.pid_s = S.alloc<int>(P);
+  d.xa = S.alloc<float4>(P);
+  d.xb = S.alloc<float4>(P);
+  d.xold = S.alloc<float4>(P);
+  d.lam = S.alloc<float>(P);
+  d.rho = S.alloc<float>(P);
+  d.v = S.alloc<float4>(P);
+  if (sh.vorticity) {
+    d.vx = S.alloc<float4>(P);
+    d.om = S.alloc<float4>(P);
+  }
+  d.nbr = S.alloc<int>(P);
+  d.red = S.alloc<double>((P / 64 + 1) * 8);
+  d.err = S.alloc<int>(1);
+  d.block_scene = S.alloc<int>(S.block_scene.size());
+  d.scenes = S.alloc<pbf::Scene>(nscenes);
+  S.diag_out = S.alloc<double>(8);
+  PBF_HIP(hipHostMalloc((void **)&S.scenes_pinned,
+                        nscenes * sizeof(pbf::Scene), hipHostMallocDefault));
+
+  // Pack the particles as Simulation::init does (Simulation.cpp:120-134):
+  // pos = (x, 0), vel = (v, m); slots start in input order.
+  std::vector<float4> hpos(P, make_float4(0, 0, 0, 0)), hvel(P, hpos[0]);
+  std::vector<int> hpid(P, -1);
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    const pbf::Scene &sc = S.scenes[s];
+    for (uint32_t k = 0; k < counts[s]; ++k) {
+      const pbf_particle &q = parts[s][k];
+      hpos[sc.slot_off + k] = make_float4(q.x[0], q.x[1], q.x[2], 0.0f);
+      hvel[sc.slot_off + k] = make_float4(q.v[0], q.v[1], q.v[2], q.m);
+      hpid[sc.slot_off + k] = sc.id_base + (int)k;
+    }
+  }
+  PBF_HIP(hipMemcpy(d.pos, hpos.data(), P * sizeof(float4),
+                    hipMemcpyHostToDevice));
+  PBF_HIP(hipMemcpy(d.vel, hvel.data(), P * sizeof(float4),
+                    hipMemcpyHostToDevice));
+  PBF_HIP(hipMemcpy(d.pid, hpid.data(), P * sizeof(int),
+                    hipMemcpyHostToDevice));
+  PBF_HIP(hipMemset(d.count, 0, M * sizeof(int)));
+  PBF_HIP(hipMemset(d.err, 0, sizeof(int)));
+  PBF_HIP(hipMemcpy(d.block_scene, S.block_scene.data(),
+                    S.block_scene.size() * sizeof(int),
+                    hipMemcpyHostToDevice));
+  S.upload_scenes();
+  PBF_HIP(hipStreamSynchronize(S.stream));
+}
+
+void ensure_graph(pbf_sim &S) {
+  if (S.graph) return;
+  hipGraph_t g = nullptr;
+  PBF_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+  pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);
+  PBF_HIP(hipStreamEndCapture(S.stream, &g));
+  const hipError_t e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    throw ApiError(PBF_ERR_HIP, std::string("hipGraphInstantiate: ") +
+                                    hipGetErrorString(e));
+}
+
+void need(bool c, const char *m) {
+  if (!c) invalid(m);
+}
+
+struct Host {
+  std::vector<float4> pos, vel, x;
+  std::vector<int> pid, ids, key, nbr;
+  std::vector<float> a, b;
+};
+
+}  // namespace
+
+extern "C" {
+
+int pbf_params_init(pbf_params *p, int32_t physics) {
+  return guard([&] {
+    need(p != nullptr, "params is NULL");
+    if (physics != PBF_PHYSICS_REF && physics != PBF_PHYSICS_PAPER)
+      invalid("unknown physics mode");
+    std::memset(p, 0, sizeof *p);
+    p->physics = physics;
+    p->iterations = 4;           // Simulation.cpp:503
+    p->gravity_y = -9.81f;       // predict_positions.cl:8
+    p->xsph_c = 0.01f;           // apply_vorticity_and_viscosity.cl:103
+    p->precision = PBF_PRECISION_FP32;
+    p->fp32_iterations = 1;
+    if (physics == PBF_PHYSICS_REF) {
+      p->relaxation = 10000.0f;  // compute_scaling.cl:16
+    } else {
+      p->relaxation = 600.0f;
+      // the stable decade at rho0 = 6378, h = 0.1, m = 1, eps = 600; k = 0.1
+      // diverges there (the artificial pressure at rest spacing is ~200x
+      // lambda)
+      p->scorr_k = 0.001f;
+      p->scorr_n = 4.0f;
+      p->scorr_dq = 0.2f;
+      p->vorticity_eps = 1e-4f;
+    }
+  });
+}
+
+int pbf_create(const pbf_params *params, const pbf_particle *particles,
+               uint32_t n, pbf_sim **out) {
+  return guard([&] {
+    need(params != nullptr && out != nullptr, "params/out is NULL");
+    *out = nullptr;
+    auto *S = new pbf_sim();
+    try {
+      const pbf_particle *parts[1] = {particles};
+      build(*S, 1, params, parts, &n);
+    } catch (...) {
+      delete S;
+      throw;
+    }
+    *out = S;
+  });
+}
+
+int pbf_batch_create(uint32_t nscenes, const pbf_params *params,
+                     const pbf_particle *const *particles,
+                     const uint32_t *counts, pbf_sim **out) {
+  return guard([&] {
+    need(params && particles && counts && out, "NULL argument");
+    *out = nullptr;
+    auto *S = new pbf_sim();
+    try {
+      build(*S, nscenes, params, particles, counts);
+    } catch (...) {
+      delete S;
+      throw;
+    }
+    *out = S;
+  });
+}
+
+int pbf_step(pbf_sim *S, uint32_t nsteps) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    if (S->shape.slots == 0 || nsteps == 0) return;
+    PBF_HIP(hipSetDevice(S->device));
+    S->upload_scenes();
+    ensure_graph(*S);
+    for (uint32_t t = 0; t < nsteps; ++t)
+      PBF_HIP(hipGraphLaunch(S->graph, S->stream));
+  });
+}
+
+int pbf_synchronize(pbf_sim *S) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    PBF_HIP(hipStreamSynchronize(S->stream));
+    S->check_err();
+  });
+}
+
+static void dump_scene(pbf_sim &S, uint32_t s, float *pos4, float *vel4) {
+  const pbf::Scene &sc = S.scenes[s];
+  const size_t n = (size_t)sc.n;
+  if (n == 0) return;
+  const size_t padded = (n + pbf::kSlotAlign - 1) / pbf::kSlotAlign *
+                        pbf::kSlotAlign;
+  std::vector<float4> hp(padded), hv(padded);
+  std::vector<int> hid(padded);
+  PBF_HIP(hipMemcpyAsync(hp.data(), S.d.pos + sc.slot_off,
+                         padded * sizeof(float4), hipMemcpyDeviceToHost,
+                         S.stream));
+  PBF_HIP(hipMemcpyAsync(hv.data(), S.d.vel + sc.slot_off,
+                         padded * sizeof(float4), hipMemcpyDeviceToHost,
+                         S.stream));
+  PBF_HIP(hipMemcpyAsync(hid.data(), S.d.pid + sc.slot_off,
+                         padded * sizeof(int), hipMemcpyDeviceToHost,
+                         S.stream));
+  PBF_HIP(hipStreamSynchronize(S.stream));
+  for (size_t k = 0; k < n; ++k) {
+    const int id = hid[k] - sc.id_base;
+    if (id < 0 || (size_t)id >= n)
+      throw ApiError(PBF_ERR_STATE, "corrupt particle id table");
+    if (pos4) std::memcpy(pos4 + 4 * (size_t)id, &hp[k], sizeof(float4));
+    if (vel4) std::memcpy(vel4 + 4 * (size_t)id, &hv[k], sizeof(float4));
+  }
+}
+
+int pbf_dump(pbf_sim *S, float *pos4, float *vel4) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    need(S->scenes.size() == 1, "pbf_dump needs a single-scene handle; use "
+                                "pbf_batch_dump");
+    PBF_HIP(hipSetDevice(S->device));
+    dump_scene(*S, 0, pos4, vel4);
+    S->check_err();
+  });
+}
+
+int pbf_batch_dump(pbf_sim *S, uint32_t scene, float *pos4, float *vel4) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    need(scene < S->scenes.size(), "scene index out of range");
+    PBF_HIP(hipSetDevice(S->device));
+    dump_scene(*S, scene, pos4, vel4);
+    S->check_err();
+  });
+}
+
+int pbf_set_wave_generator(pbf_sim *S, float value) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    S->wave = value;
+    for (auto &sc : S->scenes) sc.wave = value;
+    S->dirty = true;
+  });
+}
+
+int pbf_get_box(const pbf_sim *S, float min4[4], float max4[4]) {
+  return guard([&] {
+    need(S && min4 && max4, "NULL argument");
+    const pbf_params &p = S->params[0];
+    const float mn[4] = {p.x_min, p.y_min, p.z_min, 0.0f};
+    const float mx[4] = {p.x_max, p.y_max, p.z_max, 0.0f};
+    std::memcpy(min4, mn, sizeof mn);
+    std::memcpy(max4, mx, sizeof mx);
+  });
+}
+
+uint32_t pbf_num_particles(const pbf_sim *S) {
+  return S ? (uint32_t)S->total : 0u;
+}
+
+uint32_t pbf_num_scenes(const pbf_sim *S) {
+  return S ? (uint32_t)S->scenes.size() : 0u;
+}
+
+uint32_t pbf_num_cells(const pbf_sim *S) {
+  return S ? (uint32_t)S->shape.cells : 0u;
+}
+
+int pbf_destroy(pbf_sim *S) {
+  return guard([&] {
+    if (S) {
+      (void)hipSetDevice(S->device);
+      delete S;
+    }
+  });
+}
+
+const char *pbf_last_error(void) { return g_error.c_str(); }
+
+int pbf_probe(pbf_sim *S, int32_t *cell_of, int32_t *sorted_ids,
+              int32_t *cell_start, int32_t *nbr_count, float *rho,
+              float *lambda, float *pred4) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    need(S->scenes.size() == 1, "pbf_probe needs a single-scene handle");
+    const pbf::Scene &sc = S->scenes[0];
+    const size_t n = (size_t)sc.n;
+    if (n == 0) {
+      if (cell_start)
+        std::fill(cell_start, cell_start + S->shape.cells + 1, 0);
+      return;
+    }
+    PBF_HIP(hipSetDevice(S->device));
+    S->upload_scenes();
+    pbf::enqueue_probe(S->d, S->shape, S->stream);
+    const size_t P = (size_t)S->shape.slots, M = (size_t)S->shape.cells;
+    Host h;
+    h.pid.resize(P);
+    h.key.resize(P);
+    h.ids.resize(P);
+    h.nbr.resize(P);
+    h.a.resize(P);
+    h.b.resize(P);
+    h.x.resize(P);
+    std::vector<int> cs(M + 1);
+    auto d2h = [&](void *dst, const void *src, size_t bytes) {
+      PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                             S->stream));
+    };
+    d2h(h.pid.data(), S->d.pid, P * sizeof(int));
+    d2h(h.key.data(), S->d.key, P * sizeof(int));
+    d2h(h.ids.data(), S->d.pid_s, P * sizeof(int));
+    d2h(h.nbr.data(), S->d.nbr, P * sizeof(int));
+    d2h(h.a.data(), S->d.rho, P * sizeof(float));
+    d2h(h.b.data(), S->d.lam, P * sizeof(float));
+    d2h(h.x.data(), S->d.xa, P * sizeof(float4));
+    d2h(cs.data(), S->d.cell_start, (M + 1) * sizeof(int));
+    PBF_HIP(hipStreamSynchronize(S->stream));
+    for (size_t k = 0; k < n; ++k) {
+      const int cur = h.pid[k] - sc.id_base;    // previous order
+      const int srt = h.ids[k] - sc.id_base;    // sorted order
+      if (cur < 0 || (size_t)cur >= n || srt < 0 || (size_t)srt >= n)
+        throw ApiError(PBF_ERR_STATE, "corrupt particle id table");
+      if (cell_of) cell_of[cur] = h.key[k] - sc.cell_base;
+      if (sorted_ids) sorted_ids[k] = srt;
+      if (nbr_count) nbr_count[srt] = h.nbr[k];
+      if (rho) rho[srt] = h.a[k];
+      if (lambda) lambda[srt] = h.b[k];
+      if (pred4) std::memcpy(pred4 + 4 * (size_t)srt, &h.x[k], sizeof(float4));
+    }
+    if (cell_start) std::copy(cs.begin(), cs.end(), cell_start);
+  });
+}
+
+int pbf_diagnostics(pbf_sim *S, double out[8]) {
+  return guard([&] {
+    need(S && out, "NULL argument");
+    std::fill(out, out + 8, 0.0);
+    if (S->shape.slots == 0) return;
+    PBF_HIP(hipSetDevice(S->device));
+    S->upload_scenes();
+    pbf::enqueue_diag(S->d, S->shape, S->stream, S->diag_out);
+    double h[8];
+    PBF_HIP(hipMemcpyAsync(h, S->diag_out, sizeof h, hipMemcpyDeviceToHost,
+                           S->stream));
+    PBF_HIP(hipStreamSynchronize(S->stream));
+    // out: max (rho/rho0-1)+, mean |rho/rho0-1|, momentum xyz, energy, count
+    out[0] = h[0];
+    out[1] = h[2] > 0 ? h[1] / h[2] : 0.0;
+    out[2] = h[3];
+    out[3] = h[4];
+    out[4] = h[5];
+    out[5] = h[6];
+    out[6] = h[2];
+    out[7] = 0.0;
+  });
+}
+
+int pbf_profile_steps(pbf_sim *S, uint32_t nsteps, double *ms,
+                      uint32_t *launches, const char **names,
+                      uint32_t *nclass) {
+  return guard([&] {
+    need(S && ms && launches && nclass, "NULL argument");
+    const uint32_t cap = *nclass;
+    *nclass = pbf::kNumClasses;
+    for (uint32_t k = 0; k < cap && k < (uint32_t)pbf::kNumClasses; ++k) {
+      ms[k] = 0.0;
+      launches[k] = 0;
+      if (names) names[k] = pbf::kernel_class_name((int)k);
+    }
+    if (S->shape.slots == 0) return;
+    PBF_HIP(hipSetDevice(S->device));
+    S->upload_scenes();
+    EventTimer timer;
+    for (uint32_t t = 0; t < nsteps; ++t)
+      pbf::enqueue_step(S->d, S->shape, S->stream, &timer);
+    PBF_HIP(hipStreamSynchronize(S->stream));
+    for (const auto &r : timer.recs) {
+      float e = 0.0f;
+      PBF_HIP(hipEventElapsedTime(&e, r.a, r.b));
+      if ((uint32_t)r.cls < cap) {
+        ms[r.cls] += e;
+        launches[r.cls] += 1;
+      }
+    }
+  });
+}
+
+int pbf_time_steps(pbf_sim *S, uint32_t nsteps, double *ms) {
+  return guard([&] {
+    need(S && ms, "NULL argument");
+    *ms = 0.0;
+    if (S->shape.slots == 0 || nsteps == 0) return;
+    PBF_HIP(hipSetDevice(S->device));
+    S->upload_scenes();
+    ensure_graph(*S);
+    hipEvent_t a, b;
+    PBF_HIP(hipEventCreate(&a));
+    PBF_HIP(hipEventCreate(&b));
+    PBF_HIP(hipEventRecord(a, S->stream));
+    for (uint32_t t = 0; t < nsteps; ++t)
+      PBF_HIP(hipGraphLaunch(S->graph, S->stream));
+    PBF_HIP(hipEventRecord(b, S->stream));
+    PBF_HIP(hipEventSynchronize(b));
+    float e = 0.0f;
+    PBF_HIP(hipEventElapsedTime(&e, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = e;
+  });
+}
+
+int pbf_get_config(const pbf_sim *S, int32_t cfg[4]) {
+  return guard([&] {
+    need(S && cfg, "NULL argument");
+    cfg[0] = S->shape.block;
+    cfg[1] = 1;
+    cfg[2] = 0;
+    cfg[3] = S->shape.fp16_iterations;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,937 @@
+// pbf_kernels.hip - hand-written CDNA4 (gfx950) kernels of one PBF step.
+//
+// Layout: particles live in slots sorted by (cell, particle id); each
+// scene's slots start at a multiple of kSlotAlign.  A cell table
+// cell_start[M+1] (exclusive scan of per-cell counts, unpadded) gives every
+// cell's slot range, and because cell ids run x-fastest the three x-cells of
+// one (y, z) row of the 27-cell stencil are ONE contiguous slot range:
+// every gather walks 9 ranges instead of 27 lists.
+//
+// Step (reference sequence src/Simulation.cpp:455-576):
+//   predict_count  predict (predict_positions.cl:5-9), cell id
+//                  (update_cells.cl:9-15), per-cell arrival rank (atomic)
+//   scan x3        exclusive scan of counts -> cell_start, counts zeroed
+//   scatter        slot = cell_start + rank   (arrival order)
+//   reorder        in-cell rank by particle id -> deterministic (cell, id)
+//                  order; gathers x*, x, id into sorted slots
+//   I x (lambda, delta)   27-cell gathers (compute_scaling.cl,
+//                  compute_delta.cl + update_predicted.cl); the last delta
+//                  pass also forms v = (x* - x)/dt (update_velocities.cl)
+//   xsph_final     XSPH (apply_vorticity_and_viscosity.cl) fused with
+//                  update_positions.cl; PAPER with vorticity splits it into
+//                  xsph_omega + vort_final.
+#include "pbf_kernels.hpp"
+
+#include <algorithm>
+
+namespace pbf {
+
+#define PBF_REF_EXACT _Pragma("clang fp contract(off)")
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// Work-groups are dispatched round-robin over the 8 XCDs.  Remap so that
+// each XCD walks one contiguous chunk of slots: spatially adjacent
+// particles (which share stencil rows) then hit the same XCD's L2.
+__device__ __forceinline__ int logical_block() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
+__device__ __forceinline__ int scene_of(const int *__restrict__ block_scene,
+                                        int lb) {
+  return __builtin_amdgcn_readfirstlane(
+      block_scene[(lb * (int)blockDim.x) / kSlotAlign]);
+}
+
+// The kernels' "(int)(float)" made total: NaN -> 0, bounded magnitude.
+__device__ __forceinline__ int trunc_int(float v) {
+  if (v != v) return 0;
+  v = fminf(fmaxf(v, -1e9f), 1e9f);
+  return (int)v;
+}
+
+// Cell coordinates of a position.  REF: (int)((x - SYSTEM_MIN) /
+// CELL_LENGTH) with IEEE division (update_cells.cl:9-15).  PAPER: the same
+// with a multiply by the reciprocal.
+template <int PH>
+__device__ __forceinline__ void cell_coords(const Scene &sc, float x, float y,
+                                            float z, int &cx, int &cy,
+                                            int &cz) {
+  if constexpr (PH == kRef) {
+    cx = trunc_int((x - sc.bmin[0]) / sc.cell_len[0]);
+    cy = trunc_int((y - sc.bmin[1]) / sc.cell_len[1]);
+    cz = trunc_int((z - sc.bmin[2]) / sc.cell_len[2]);
+  } else {
+    cx = trunc_int((x - sc.bmin[0]) * sc.inv_cell[0]);
+    cy = trunc_int((y - sc.bmin[1]) * sc.inv_cell[1]);
+    cz = trunc_int((z - sc.bmin[2]) * sc.inv_cell[2]);
+  }
+}
+
+// Visit every slot of the 27-cell stencil of a particle at xi (cells from
+// its CURRENT position, lists from the step's sort, neighbour cells outside
+// the grid skipped: compute_scaling.cl:21-53).  Rows (dz, dy) each give one
+// contiguous range over the x-cells [cx-1, cx+1] clipped to the grid.
+template <int PH, typename F>
+__device__ __forceinline__ void for_stencil(const Scene &sc,
+                                            const int *__restrict__ cs,
+                                            float x, float y, float z, F &&f) {
+  int cx, cy, cz;
+  cell_coords<PH>(sc, x, y, z, cx, cy, cz);
+  const int xlo = max(cx - 1, 0), xhi = min(cx + 1, sc.nc[0] - 1);
+  if (xlo > xhi) return;
+  const int ylo = max(cy - 1, 0), yhi = min(cy + 1, sc.nc[1] - 1);
+  const int zlo = max(cz - 1, 0), zhi = min(cz + 1, sc.nc[2] - 1);
+  for (int zz = zlo; zz <= zhi; ++zz) {
+    for (int yy = ylo; yy <= yhi; ++yy) {
+      const int base = sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0];
+      const int b = cs[base + xlo] + sc.shift;
+      const int e = cs[base + xhi + 1] + sc.shift;
+      for (int j = b; j < e; ++j) f(j);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// REF pair terms, in the reference kernels' operation order, no contraction
+// ---------------------------------------------------------------------------
+
+// predictPositions (predict_positions.cl:8-9).
+__device__ __forceinline__ float3 predict_exact(const Scene &sc, float4 x,
+                                                float4 &v) {
+  PBF_REF_EXACT
+  v.x = v.x + sc.dt * 0.0f;
+  v.y = v.y + sc.dt * sc.g_y;
+  v.z = v.z + sc.dt * 0.0f;
+  return make_float3(x.x + sc.dt * v.x, x.y + sc.dt * v.y, x.z + sc.dt * v.z);
+}
+
+// r^2 left to right without fusing (compute_scaling.cl:58-59).
+__device__ __forceinline__ float dist2_exact(float3 xi, float4 xj, float &rx,
+                                             float &ry, float &rz) {
+  PBF_REF_EXACT
+  rx = xi.x - xj.x;
+  ry = xi.y - xj.y;
+  rz = xi.z - xj.z;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// compute_scaling.cl:56-84 for one list entry.
+__device__ __forceinline__ void ref_lambda_pair(const Scene &sc, float3 xi,
+                                                float4 xj, float &dens,
+                                                float &gsum, float &gix,
+                                                float &giy, float &giz) {
+  PBF_REF_EXACT
+  float rx, ry, rz;
+  const float r2 = dist2_exact(xi, xj, rx, ry, rz);
+  if (r2 > 0.0f && r2 < sc.h2) {
+    const float rl = sqrtf(r2);
+    const float gx = rx / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gy = ry / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gz = rz / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float w = sc.poly6 * (sc.h2 - r2) * (sc.h2 - r2) * (sc.h2 - r2);
+    dens += w;
+    gsum += sqrtf(gx * gx + gy * gy + gz * gz);
+    gix += gx;
+    giy += gy;
+    giz += gz;
+  }
+}
+
+// compute_scaling.cl:129-140.
+__device__ __forceinline__ float ref_lambda_final(const Scene &sc, float dens,
+                                                  float gsum, float gix,
+                                                  float giy, float giz) {
+  PBF_REF_EXACT
+  gsum += sqrtf(gix * gix + giy * giy + giz * giz);
+  const float c = (dens / sc.rho0) - 1.0f;
+  return -1.0f * c / (gsum * gsum / (sc.rho0 * sc.rho0) + sc.eps);
+}
+
+// compute_delta.cl:56-80 for one list entry.
+__device__ __forceinline__ void ref_delta_pair(const Scene &sc, float3 xi,
+                                               float4 xj, float li, float lj,
+                                               float &sx, float &sy,
+                                               float &sz) {
+  PBF_REF_EXACT
+  float rx, ry, rz;
+  const float r2 = dist2_exact(xi, xj, rx, ry, rz);
+  if (r2 > 0.0f && r2 < sc.h2) {
+    const float rl = sqrtf(r2);
+    const float gx = -1.0f * rx / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gy = -1.0f * ry / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gz = -1.0f * rz / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float s = li + lj;
+    sx += s * gx;
+    sy += s * gy;
+    sz += s * gz;
+  }
+}
+
+// compute_delta.cl:125-152 then update_predicted.cl:7: future = x* + dp,
+// wall clamp (x-min wall moved by the wave generator), PAPER sphere first;
+// returns x* + (future - x*).
+__device__ __forceinline__ float3 clamp_update(const Scene &sc, float3 xi,
+                                               float3 dp, bool sphere) {
+  PBF_REF_EXACT
+  float fx = xi.x + dp.x, fy = xi.y + dp.y, fz = xi.z + dp.z;
+  if (sphere) {
+    const float dx = fx - sc.sphere[0], dy = fy - sc.sphere[1],
+                dz = fz - sc.sphere[2];
+    const float rr = sc.sphere[3] + sc.h;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < rr * rr) {
+      if (d2 > 0.0f) {
+        const float s = rr / sqrtf(d2);
+        fx = sc.sphere[0] + dx * s;
+        fy = sc.sphere[1] + dy * s;
+        fz = sc.sphere[2] + dz * s;
+      } else {
+        fy = sc.sphere[1] + rr;
+      }
+    }
+  }
+  if ((fx - sc.h) < (sc.bmin[0] + sc.wave)) {
+    fx = sc.bmin[0] + sc.wave + sc.h;
+  } else if ((fx + sc.h) > sc.bmax[0]) {
+    fx = sc.bmax[0] - sc.h;
+  }
+  if ((fy - sc.h) < sc.bmin[1]) {
+    fy = sc.bmin[1] + sc.h;
+  } else if ((fy + sc.h) > sc.bmax[1]) {
+    fy = sc.bmax[1] - sc.h;
+  }
+  if ((fz - sc.h) < sc.bmin[2]) {
+    fz = sc.bmin[2] + sc.h;
+  } else if ((fz + sc.h) > sc.bmax[2]) {
+    fz = sc.bmax[2] - sc.h;
+  }
+  const float dx = fx - xi.x, dy = fy - xi.y, dz = fz - xi.z;
+  return make_float3(xi.x + dx, xi.y + dy, xi.z + dz);
+}
+
+// apply_vorticity_and_viscosity.cl:56-61 for one list entry.
+__device__ __forceinline__ void ref_xsph_pair(const Scene &sc, float3 xi,
+                                              float4 xj, float4 vi, float4 vj,
+                                              float &sx, float &sy,
+                                              float &sz) {
+  PBF_REF_EXACT
+  float rx, ry, rz;
+  const float r2 = dist2_exact(xi, xj, rx, ry, rz);
+  if (r2 > 0.0f && r2 < sc.h2) {
+    const float w = sc.poly6 * (sc.h2 - r2) * (sc.h2 - r2) * (sc.h2 - r2);
+    const float inv = 1.0f / xj.w;
+    sx += inv * (vj.x - vi.x) * w;
+    sy += inv * (vj.y - vi.y) * w;
+    sz += inv * (vj.z - vi.z) * w;
+  }
+}
+
+// update_velocities.cl:8.
+__device__ __forceinline__ float3 velocity_exact(const Scene &sc, float3 xs,
+                                                 float4 xo) {
+  PBF_REF_EXACT
+  return make_float3((xs.x - xo.x) / sc.dt, (xs.y - xo.y) / sc.dt,
+                     (xs.z - xo.z) / sc.dt);
+}
+
+// update_positions.cl:10-12: v += dv, |v|.
+__device__ __forceinline__ float4 add_len_exact(float4 v, float dx, float dy,
+                                                float dz, float &len) {
+  PBF_REF_EXACT
+  v.x = v.x + dx;
+  v.y = v.y + dy;
+  v.z = v.z + dz;
+  len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  return v;
+}
+
+__device__ __forceinline__ float4 mul_exact(float c, float x, float y,
+                                            float z) {
+  PBF_REF_EXACT
+  return make_float4(c * x, c * y, c * z, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// neighbour search
+// ---------------------------------------------------------------------------
+
+template <int PH>
+__global__ __launch_bounds__(256) void k_predict_count(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+    float4 *__restrict__ pred_u, int *__restrict__ key,
+    int *__restrict__ rank, int *__restrict__ count, int predict) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x = pos[i];
+  float4 v = vel[i];
+  float3 p = make_float3(x.x, x.y, x.z);
+  if (predict) p = predict_exact(sc, x, v);
+  int cx, cy, cz;
+  cell_coords<PH>(sc, p.x, p.y, p.z, cx, cy, cz);
+  cx = min(max(cx, 0), sc.nc[0] - 1);
+  cy = min(max(cy, 0), sc.nc[1] - 1);
+  cz = min(max(cz, 0), sc.nc[2] - 1);
+  const int k = sc.cell_base + cx + (cy + cz * sc.nc[1]) * sc.nc[0];
+  key[i] = k;
+  rank[i] = atomicAdd(&count[k], 1);
+  pred_u[i] = make_float4(p.x, p.y, p.z, v.w);
+}
+
+// Exclusive scan over a 256-thread block; *total receives the block sum.
+__device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[w] = x;
+  __syncthreads();
+  int before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = lds[k];
+    before += (k < w) ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(
+    const int *__restrict__ count, int m, int *__restrict__ partial) {
+  __shared__ int lds[4];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanIpt;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanIpt; ++k) {
+    const int idx = base + k;
+    s += (idx < m) ? count[idx] : 0;
+  }
+  int tot;
+  block_excl_scan256(s, lds, tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// One block: exclusive scan of the block sums in place; writes the grand
+// total to cell_start[m].
+__global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ partial,
+                                                  int nb,
+                                                  int *__restrict__ cell_start,
+                                                  int m) {
+  __shared__ int lds[4];
+  int carry = 0;
+  for (int base = 0; base < nb; base += 256 * kScanIpt) {
+    int v[kScanIpt];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanIpt; ++k) {
+      const int idx = base + threadIdx.x * kScanIpt + k;
+      v[k] = (idx < nb) ? partial[idx] : 0;
+      s += v[k];
+    }
+    int tot;
+    int run = block_excl_scan256(s, lds, tot) + carry;
+#pragma unroll
+    for (int k = 0; k < kScanIpt; ++k) {
+      const int idx = base + threadIdx.x * kScanIpt + k;
+      if (idx < nb) partial[idx] = run;
+      run += v[k];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) cell_start[m] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(
+    int *__restrict__ count, int m, const int *__restrict__ partial,
+    int *__restrict__ cell_start) {
+  __shared__ int lds[4];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanIpt;
+  int v[kScanIpt];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanIpt; ++k) {
+    const int idx = base + k;
+    v[k] = (idx < m) ? count[idx] : 0;
+    s += v[k];
+  }
+  int tot;
+  int run = block_excl_scan256(s, lds, tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanIpt; ++k) {
+    const int idx = base + k;
+    if (idx < m) {
+      cell_start[idx] = run;
+      count[idx] = 0;
+    }
+    run += v[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const int *__restrict__ key, const int *__restrict__ rank,
+    const int *__restrict__ pid, const int *__restrict__ cell_start,
+    int *__restrict__ perm, int *__restrict__ ppid) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const int slot = cell_start[key[i]] + rank[i] + sc.shift;
+  perm[slot] = i;
+  ppid[slot] = pid[i];
+}
+
+// Arrival order within a cell is not deterministic; rank each slot among
+// its cell by particle id and move it there, gathering x*, x and the id.
+__global__ __launch_bounds__(256) void k_reorder(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const int *__restrict__ perm, const int *__restrict__ ppid,
+    const int *__restrict__ key, const int *__restrict__ cell_start,
+    const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
+    int *__restrict__ pid_s, float4 *__restrict__ xa,
+    float4 *__restrict__ xold) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int t = lb * blockDim.x + threadIdx.x;
+  if (t - sc.slot_off >= sc.n) return;
+  const int src = perm[t];
+  const int p = ppid[t];
+  const int c = key[src];
+  const int lo = cell_start[c] + sc.shift, hi = cell_start[c + 1] + sc.shift;
+  int r = 0;
+  for (int u = lo; u < hi; ++u) r += (ppid[u] < p) ? 1 : 0;
+  const int f = lo + r;
+  const float4 pu = pred_u[src];
+  const float4 x = pos[src];
+  pid_s[f] = p;
+  xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+  xold[f] = make_float4(x.x, x.y, x.z, pu.w);
+}
+
+// ---------------------------------------------------------------------------
+// constraint solve
+// ---------------------------------------------------------------------------
+
+// compute_scaling.cl:12-140 (REF) / eqs. 1, 2, 8, 9, 11 (PAPER).
+template <int PH>
+__global__ __launch_bounds__(256) void k_lambda(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ X, const int *__restrict__ cs,
+    float *__restrict__ lam, float *__restrict__ rho) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = X[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  if constexpr (PH == kRef) {
+    float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
+    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      if (j != i) ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
+    });
+    lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
+    rho[i] = dens;
+  } else {
+    // sums of (h^2-r^2)^3, (h-r)^4 and (h-r)^2 r/|r|; constants applied once
+    float wsum = 0.0f, g2 = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    const float h = sc.h, h2 = sc.h2;
+    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      const float4 xj = X[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 > 0.0f && r2 < h2) {
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float hr = h - r2 * inv;
+        const float t = h2 - r2;
+        const float hr2 = hr * hr;
+        wsum += t * t * t;
+        g2 += hr2 * hr2;
+        const float gs = hr2 * inv;
+        gx += rx * gs;
+        gy += ry * gs;
+        gz += rz * gs;
+      }
+    });
+    const float dens = sc.poly6 * wsum;
+    const float sp2 = sc.spiky * sc.spiky;
+    const float denom =
+        sp2 * (gx * gx + gy * gy + gz * gz + g2) * (sc.inv_rho0 * sc.inv_rho0) +
+        sc.eps;
+    lam[i] = -(dens * sc.inv_rho0 - 1.0f) / denom;
+    rho[i] = dens;
+  }
+}
+
+// compute_delta.cl:13-152 + update_predicted.cl (REF) / eqs. 13, 14
+// (PAPER).  LAST also forms v = (x* - x)/dt (update_velocities.cl:8) and
+// carries rho of the last lambda pass in .w for the XSPH pass.
+template <int PH, bool LAST>
+__global__ __launch_bounds__(256) void k_delta(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ X, const int *__restrict__ cs,
+    const float *__restrict__ lam, const float *__restrict__ rho,
+    const float4 *__restrict__ xold, float4 *__restrict__ Xn,
+    float4 *__restrict__ V) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = X[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  const float li = lam[i];
+  float3 dp;
+  if constexpr (PH == kRef) {
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      if (j != i) ref_delta_pair(sc, xi, X[j], li, lam[j], sx, sy, sz);
+    });
+    dp = make_float3(sx / sc.rho0, sy / sc.rho0, sz / sc.rho0);
+  } else {
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
+    const int n = sc.scorr_n;
+    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      const float4 xj = X[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 > 0.0f && r2 < h2) {
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float hr = h - r2 * inv;
+        const float t = h2 - r2;
+        const float q = t * t * t * cq;  // W / W(dq)
+        float qn = 1.0f;
+        for (int e = 0; e < n; ++e) qn *= q;
+        const float coef = li + lam[j] - kk * qn;
+        const float gs = coef * hr * hr * inv;
+        ax += rx * gs;
+        ay += ry * gs;
+        az += rz * gs;
+      }
+    });
+    // grad_i W = -spiky (h-r)^2 r/|r|
+    const float c = -sc.spiky * sc.inv_rho0;
+    dp = make_float3(ax * c, ay * c, az * c);
+  }
+  const float3 xn = clamp_update(sc, xi, dp, PH == kPaper && sc.sphere[3] > 0.0f);
+  Xn[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
+  if constexpr (LAST) {
+    const float4 xo = xold[i];
+    const float3 v = velocity_exact(sc, xn, xo);
+    V[i] = make_float4(v.x, v.y, v.z, xo.w);
+  }
+}
+
+// Final write of a slot: x = x*, v, |v| (update_positions.cl:9-12), id.
+__device__ __forceinline__ void write_final(int i, float3 xs, float4 v,
+                                            float len, float4 *__restrict__ pos,
+                                            float4 *__restrict__ vel,
+                                            int *__restrict__ pid,
+                                            const int *__restrict__ pid_s,
+                                            int *__restrict__ err) {
+  pos[i] = make_float4(xs.x, xs.y, xs.z, len);
+  vel[i] = v;
+  pid[i] = pid_s[i];
+  if (!(xs.x == xs.x && xs.y == xs.y && xs.z == xs.z && len == len))
+    atomicOr(err, 1);
+}
+
+// apply_vorticity_and_viscosity.cl:12-104 fused with update_positions.cl
+// (REF, and PAPER without vorticity).
+template <int PH>
+__global__ __launch_bounds__(256) void k_xsph_final(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ X, const int *__restrict__ cs,
+    const float4 *__restrict__ V, const int *__restrict__ pid_s,
+    float4 *__restrict__ pos, float4 *__restrict__ vel,
+    int *__restrict__ pid, int *__restrict__ err) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = X[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  const float4 vi = V[i];
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  if constexpr (PH == kRef) {
+    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      if (j != i) ref_xsph_pair(sc, xi, X[j], vi, V[j], sx, sy, sz);
+    });
+  } else {
+    const float h2 = sc.h2;
+    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      const float4 xj = X[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 > 0.0f && r2 < h2) {
+        const float4 vj = V[j];
+        const float t = h2 - r2;
+        // a neighbour with no density yet (entered the support after the
+        // last lambda pass) carries no weight
+        const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
+        sx += (vj.x - vi.x) * w;
+        sy += (vj.y - vi.y) * w;
+        sz += (vj.z - vi.z) * w;
+      }
+    });
+    const float c = sc.poly6;
+    sx *= c;
+    sy *= c;
+    sz *= c;
+  }
+  const float4 dv = mul_exact(sc.c_xsph, sx, sy, sz);
+  float len;
+  const float4 v = add_len_exact(vi, dv.x, dv.y, dv.z, len);
+  write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+}
+
+// PAPER: XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15).
+__global__ __launch_bounds__(256) void k_xsph_omega(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ X, const int *__restrict__ cs,
+    const float4 *__restrict__ V, float4 *__restrict__ VX,
+    float4 *__restrict__ OM) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = X[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  const float4 vi = V[i];
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  const float h = sc.h, h2 = sc.h2;
+  for_stencil<kPaper>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    const float4 xj = X[j];
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 > 0.0f && r2 < h2) {
+      const float4 vj = V[j];
+      const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
+      const float t = h2 - r2;
+      const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
+      sx += vx * w;
+      sy += vy * w;
+      sz += vz * w;
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float gs = hr * hr * inv;  // grad_j W_ij = spiky gs r
+      const float gx = rx * gs, gy = ry * gs, gz = rz * gs;
+      ox += vy * gz - vz * gy;
+      oy += vz * gx - vx * gz;
+      oz += vx * gy - vy * gx;
+    }
+  });
+  const float cx = sc.c_xsph * sc.poly6;
+  VX[i] = make_float4(vi.x + cx * sx, vi.y + cx * sy, vi.z + cx * sz, vi.w);
+  ox *= sc.spiky;
+  oy *= sc.spiky;
+  oz *= sc.spiky;
+  OM[i] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
+}
+
+// PAPER: eta = sum (|w_j| - |w_i|) grad_i W_ij, N = eta/|eta|,
+// v += dt eps (N x w_i) (eq. 16), then update_positions.cl.
+__global__ __launch_bounds__(256) void k_vort_final(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ X, const int *__restrict__ cs,
+    const float4 *__restrict__ VX, const float4 *__restrict__ OM,
+    const int *__restrict__ pid_s, float4 *__restrict__ pos,
+    float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = X[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  const float4 wi = OM[i];
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+  const float h = sc.h, h2 = sc.h2;
+  for_stencil<kPaper>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    const float4 xj = X[j];
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 > 0.0f && r2 < h2) {
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float gs = (OM[j].w - wi.w) * hr * hr * inv;
+      ex += rx * gs;
+      ey += ry * gs;
+      ez += rz * gs;
+    }
+  });
+  float4 v = VX[i];
+  const float e2 = ex * ex + ey * ey + ez * ez;
+  if (e2 > 0.0f) {
+    // the sign of grad_i W (-spiky) flips eta; N = -e/|e|
+    const float inv = -__builtin_amdgcn_rsqf(e2);
+    const float nx = ex * inv, ny = ey * inv, nz = ez * inv;
+    const float c = sc.dt * sc.vort_eps;
+    v.x += c * (ny * wi.z - nz * wi.y);
+    v.y += c * (nz * wi.x - nx * wi.z);
+    v.z += c * (nx * wi.y - ny * wi.x);
+  }
+  const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+}
+
+// ---------------------------------------------------------------------------
+// probes and diagnostics
+// ---------------------------------------------------------------------------
+
+// Stencil neighbour count with the reference predicate 0 < r^2 < h^2,
+// r^2 unfused (compute_scaling.cl:56-62).
+template <int PH>
+__global__ __launch_bounds__(256) void k_count_nbrs(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ X, const int *__restrict__ cs,
+    int *__restrict__ nbr) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = X[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  int c = 0;
+  for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    float rx, ry, rz;
+    const float r2 = dist2_exact(xi, X[j], rx, ry, rz);
+    if (j != i && r2 > 0.0f && r2 < sc.h2) ++c;
+  });
+  nbr[i] = c;
+}
+
+// Density at the sorted positions X, then per-block FP64 partials:
+// [max (rho/rho0-1)+, sum |rho/rho0-1|, count, px, py, pz, energy, 0].
+template <int PH>
+__global__ __launch_bounds__(256) void k_diag(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ X, const int *__restrict__ cs,
+    const float4 *__restrict__ vel, double *__restrict__ red) {
+  __shared__ double lds[8][4];
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (i - sc.slot_off < sc.n) {
+    const float4 x4 = X[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    float wsum = 0.0f;
+    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      const float4 xj = X[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      if (j != i && r2 > 0.0f && r2 < sc.h2) {
+        const float t = sc.h2 - r2;
+        wsum += t * t * t;
+      }
+    });
+    const double e = (double)(sc.poly6 * wsum) / (double)sc.rho0 - 1.0;
+    const float4 vv = vel[i];
+    const double m = vv.w;
+    v[0] = e > 0.0 ? e : 0.0;
+    v[1] = fabs(e);
+    v[2] = 1.0;
+    v[3] = m * vv.x;
+    v[4] = m * vv.y;
+    v[5] = m * vv.z;
+    v[6] = 0.5 * m * ((double)vv.x * vv.x + (double)vv.y * vv.y +
+                      (double)vv.z * vv.z);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    double x = v[k];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(x, o, 64);
+      x = (k == 0) ? fmax(x, y) : x + y;
+    }
+    v[k] = x;
+  }
+  if (lane == 0)
+    for (int k = 0; k < 8; ++k) lds[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int k = threadIdx.x;
+    const int nw = blockDim.x >> 6;
+    double x = lds[k][0];
+    for (int q = 1; q < nw; ++q) x = (k == 0) ? fmax(x, lds[k][q]) : x + lds[k][q];
+    red[(size_t)lb * 8 + k] = x;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diag_final(double *__restrict__ red,
+                                                    int nblocks,
+                                                    double *__restrict__ out) {
+  __shared__ double lds[8][4];
+  double v[8];
+  for (int k = 0; k < 8; ++k) v[k] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int k = 0; k < 8; ++k) {
+      const double x = red[(size_t)b * 8 + k];
+      v[k] = (k == 0) ? fmax(v[k], x) : v[k] + x;
+    }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < 8; ++k) {
+    double x = v[k];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(x, o, 64);
+      x = (k == 0) ? fmax(x, y) : x + y;
+    }
+    v[k] = x;
+  }
+  if (lane == 0)
+    for (int k = 0; k < 8; ++k) lds[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int k = threadIdx.x;
+    double x = lds[k][0];
+    for (int q = 1; q < 4; ++q) x = (k == 0) ? fmax(x, lds[k][q]) : x + lds[k][q];
+    out[k] = x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch sequencing
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct Launcher {
+  const StepShape &sh;
+  hipStream_t st;
+  KernelTimer *timer;
+  void begin(int cls) {
+    if (timer) timer->begin(cls, st);
+  }
+  void end(int cls) {
+    if (timer) timer->end(cls, st);
+  }
+};
+
+}  // namespace
+
+#define PBF_LAUNCH(L, cls, kern, grid, block, ...)                 \
+  do {                                                             \
+    (L).begin(cls);                                                \
+    kern<<<(grid), (block), 0, (L).st>>>(__VA_ARGS__);             \
+    (L).end(cls);                                                  \
+  } while (0)
+
+template <int PH>
+static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
+  const StepShape &sh = L.sh;
+  const int bs = 256;
+  const int grid = sh.slots / bs;
+  PBF_LAUNCH(L, kClsPredict, k_predict_count<PH>, grid, bs, d.scenes,
+             d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank, d.count,
+             predict);
+  PBF_LAUNCH(L, kClsScan, k_scan_reduce, sh.scan_blocks, 256, d.count, sh.cells,
+             d.partial);
+  PBF_LAUNCH(L, kClsScan, k_scan_top, 1, 256, d.partial, sh.scan_blocks,
+             d.cell_start, sh.cells);
+  PBF_LAUNCH(L, kClsScan, k_scan_down, sh.scan_blocks, 256, d.count, sh.cells,
+             d.partial, d.cell_start);
+  PBF_LAUNCH(L, kClsScatter, k_scatter, grid, bs, d.scenes, d.block_scene,
+             d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid);
+  PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
+             d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
+             d.xa, d.xold);
+}
+
+template <int PH>
+static void step_impl(const DevBuffers &d, Launcher &L) {
+  const StepShape &sh = L.sh;
+  const int bs = sh.block;
+  const int grid = sh.slots / bs;
+  sort_stage<PH>(d, L, 1);
+  float4 *cur = d.xa, *nxt = d.xb;
+  for (int it = 0; it < sh.iterations; ++it) {
+    const bool last = it == sh.iterations - 1;
+    PBF_LAUNCH(L, kClsLambda, k_lambda<PH>, grid, bs, d.scenes, d.block_scene,
+               cur, d.cell_start, d.lam, d.rho);
+    if (last) {
+      PBF_LAUNCH(L, kClsDelta, (k_delta<PH, true>), grid, bs, d.scenes,
+                 d.block_scene, cur, d.cell_start, d.lam, d.rho, d.xold, nxt,
+                 d.v);
+    } else {
+      PBF_LAUNCH(L, kClsDelta, (k_delta<PH, false>), grid, bs, d.scenes,
+                 d.block_scene, cur, d.cell_start, d.lam, d.rho, d.xold, nxt,
+                 d.v);
+    }
+    std::swap(cur, nxt);
+  }
+  if (PH == kPaper && sh.vorticity) {
+    PBF_LAUNCH(L, kClsXsph, k_xsph_omega, grid, bs, d.scenes, d.block_scene,
+               cur, d.cell_start, d.v, d.vx, d.om);
+    PBF_LAUNCH(L, kClsVort, k_vort_final, grid, bs, d.scenes, d.block_scene,
+               cur, d.cell_start, d.vx, d.om, d.pid_s, d.pos, d.vel, d.pid,
+               d.err);
+  } else {
+    PBF_LAUNCH(L, kClsXsph, k_xsph_final<PH>, grid, bs, d.scenes,
+               d.block_scene, cur, d.cell_start, d.v, d.pid_s, d.pos, d.vel,
+               d.pid, d.err);
+  }
+}
+
+void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
+                  KernelTimer *timer) {
+  Launcher L{sh, st, timer};
+  if (sh.physics == kRef)
+    step_impl<kRef>(d, L);
+  else
+    step_impl<kPaper>(d, L);
+}
+
+void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
+  Launcher L{sh, st, nullptr};
+  const int bs = sh.block, grid = sh.slots / bs;
+  if (sh.physics == kRef) {
+    sort_stage<kRef>(d, L, 1);
+    k_count_nbrs<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                            d.cell_start, d.nbr);
+    k_lambda<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                        d.cell_start, d.lam, d.rho);
+  } else {
+    sort_stage<kPaper>(d, L, 1);
+    k_count_nbrs<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                              d.cell_start, d.nbr);
+    k_lambda<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                          d.cell_start, d.lam, d.rho);
+  }
+}
+
+void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
+                  double *out_dev) {
+  Launcher L{sh, st, nullptr};
+  const int bs = 256, grid = sh.slots / bs;
+  if (sh.physics == kRef) {
+    sort_stage<kRef>(d, L, 0);
+    k_diag<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                      d.cell_start, d.vel, d.red);
+  } else {
+    sort_stage<kPaper>(d, L, 0);
+    k_diag<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                        d.cell_start, d.vel, d.red);
+  }
+  k_diag_final<<<1, 256, 0, st>>>(d.red, grid, out_dev);
+}
+
+const char *kernel_class_name(int cls) {
+  static const char *names[kNumClasses] = {
+      "predict_count", "scan", "scatter", "reorder", "lambda",
+      "delta",         "xsph", "vorticity"};
+  return (cls >= 0 && cls < kNumClasses) ? names[cls] : "?";
+}
+
+}  // namespace pbf

@@ -1,0 +1,55 @@
+// pbf_kernels.hpp - launch interface between the host runtime (pbf_api.cpp)
+// and the HIP kernels (pbf_kernels.hip).
+#pragma once
+
+#include "pbf_scene.hpp"
+
+namespace pbf {
+
+constexpr int kRef = 0;
+constexpr int kPaper = 1;
+
+// scan tiling: 256 threads x 16 counts per block
+constexpr int kScanIpt = 16;
+constexpr int kScanTile = 256 * kScanIpt;
+
+// kernel classes reported by the profiler
+enum KernelClass {
+  kClsPredict = 0,
+  kClsScan,
+  kClsScatter,
+  kClsReorder,
+  kClsLambda,
+  kClsDelta,
+  kClsXsph,
+  kClsVort,
+  kNumClasses
+};
+
+// Shape of one step (fixed for a handle; captured into its graph).
+struct StepShape {
+  int slots;        // padded particle slots P (multiple of kSlotAlign)
+  int cells;        // M, total cells over scenes
+  int scan_blocks;  // ceil(M / kScanTile)
+  int physics;      // kRef / kPaper (uniform over scenes)
+  int iterations;
+  int fp16_iterations;  // leading iterations gathering FP16 positions
+  int vorticity;    // PAPER: vorticity confinement pass enabled
+  int block;        // gather work-group size (64, 128 or 256)
+};
+
+// Per-kernel-class event timing (pbf_profile_steps).
+struct KernelTimer {
+  virtual void begin(int cls, hipStream_t st) = 0;
+  virtual void end(int cls, hipStream_t st) = 0;
+  virtual ~KernelTimer() = default;
+};
+
+void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
+                  KernelTimer *timer);
+void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st);
+void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
+                  double *out_dev);
+const char *kernel_class_name(int cls);
+
+}  // namespace pbf

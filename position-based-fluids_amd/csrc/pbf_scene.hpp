@@ -1,0 +1,74 @@
+// pbf_scene.hpp - per-scene constants and device buffers shared by the HIP
+// kernels (pbf_kernels.hip) and the host runtime (pbf_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace pbf {
+
+// Particle slots of every scene start at a multiple of this, so a
+// work-group of <= kSlotAlign threads never straddles two scenes and the
+// scene of a work-group is one scalar load.
+constexpr int kSlotAlign = 256;
+
+// Per-scene constants, resident in device memory and read by the kernels
+// with scalar loads (the scene index is uniform per work-group).  REF
+// scenes hold the 6-significant-digit constants of src/main.cpp:149-168.
+struct Scene {
+  float bmin[3], bmax[3];
+  float cell_len[3];   // CELL_LENGTH_* (cell coordinate = (x-min)/len)
+  float inv_cell[3];   // 1/len (PAPER cell coordinate = (x-min)*inv)
+  int nc[3];           // cells per axis
+  int cell_base;       // first global cell id of the scene
+  int ncells;          // nc[0]*nc[1]*nc[2]
+  int slot_off;        // first particle slot (multiple of kSlotAlign)
+  int n;               // particles
+  int shift;           // slot_off - (particles of the scenes before)
+  int id_base;         // particles of the scenes before (= global id base)
+  float h, h2, poly6, spiky;
+  float dt, rho0, inv_rho0, eps;
+  float c_xsph, g_y, scorr_k, scorr_cq;  // scorr_cq = poly6 / W(dq)
+  int scorr_n;
+  float vort_eps;
+  float wave;          // x-min wall offset (Simulation::setWaveGenerator)
+  float sphere[4];     // PAPER obstacle (cx, cy, cz, r), r <= 0: none
+  float half_scale;    // mixed precision: FP16 cell-relative scale
+  int pad[3];
+};
+static_assert(sizeof(Scene) % 16 == 0, "Scene must stay 16-byte aligned");
+
+// Device buffers of one handle.  Slot-indexed arrays have P entries (all
+// scenes, padded); cell-indexed arrays M (+1) entries.  "Sorted" arrays are
+// in (cell, particle id) order of the current step.
+struct DevBuffers {
+  // persistent state, sorted order of the previous step
+  float4 *pos;    // x, y, z, |v|
+  float4 *vel;    // v, m
+  int *pid;       // global original particle id
+  // step temporaries
+  float4 *pred_u; // x*, m        (previous order)
+  int *key;       // global cell id (previous order)
+  int *rank;      // arrival rank in the cell (previous order)
+  int *count;     // per cell, zero between steps
+  int *cell_start;// M+1, exclusive scan of count (unpadded slots)
+  int *partial;   // scan block sums
+  int *perm;      // slot -> previous index, arrival order
+  int *ppid;      // slot -> particle id, arrival order
+  int *pid_s;     // sorted particle id
+  float4 *xa, *xb;// x* ping-pong (sorted); .w = rho after the last pass
+  float4 *xold;   // x at step start, m (sorted)
+  float *lam;     // lambda (sorted)
+  float *rho;     // density of the last lambda pass (sorted)
+  float4 *v;      // velocity after the constraint solve (sorted), .w = m
+  float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
+  float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
+  ushort4 *xh;    // mixed precision: FP16 cell-relative x* (sorted)
+  int *nbr;       // probe: neighbour counts (sorted)
+  double *red;    // diagnostics partials
+  int *err;       // device error word
+  int *block_scene;  // per kSlotAlign block of slots -> scene
+  Scene *scenes;
+};
+
+}  // namespace pbf

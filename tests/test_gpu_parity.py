@@ -1,0 +1,295 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars:
+  * integer / index results (cell ids, sort permutation, cell table,
+    neighbour counts): bit-exact;
+  * REF physics trajectories vs the FP32 oracle (the reference kernels'
+    arithmetic): max |gpu - oracle| / max |oracle| <= 1e-5 for positions
+    and velocities after 1 and 10 steps;
+  * PAPER physics vs the FP64 oracle: positions within 1e-4 m (1e-3 h),
+    velocities within 1e-3 of the largest speed, after 10 steps;
+  * batched scenes: bit-identical to the same scenes run one by one;
+  * graph replays: bit-identical to eager launches of the same kernels.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pbf
+import pbf_oracle as orc
+import scenarios
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def threads():
+    orc.set_threads(min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def c0_ref():
+    return scenarios.config(0, physics=pbf.PHYSICS_REF)
+
+
+def run_gpu(params, parts, nsteps, wave=0.0):
+    with pbf.Simulation(params, parts) as s:
+        if wave:
+            s.set_wave_generator(wave)
+        s.step(nsteps)
+        return s.dump()
+
+
+# ----------------------------------------------------------------------------
+# neighbour search
+# ----------------------------------------------------------------------------
+
+def test_probe_c0_ref_bit_exact(c0_ref):
+    say("probe c0 ref", c0_ref.n)
+    p, parts = c0_ref.params[0], c0_ref.particles[0]
+    with pbf.Simulation(p, parts) as s:
+        g = s.probe()
+        ncell = s.num_cells()
+    pos, vel = orc.state_from_particles(parts)
+    o = orc.probe_f32(p, pos, vel, ncell)
+    assert np.array_equal(g["pred"][:, :3], o["pred"][:, :3])
+    assert np.array_equal(g["cell_of"], o["cell_of"])
+    assert np.array_equal(g["sorted_ids"], o["sorted_ids"])
+    assert np.array_equal(g["cell_start"], o["cell_start"])
+    assert np.array_equal(g["nbr_count"], o["nbr_count"])
+    assert np.allclose(g["rho"], o["rho"], rtol=1e-5)
+    lam_scale = np.abs(o["lambda"]).max()
+    assert np.abs(g["lambda"] - o["lambda"]).max() <= 1e-5 * lam_scale
+
+
+def test_probe_after_steps_bit_exact(c0_ref):
+    """After 10 GPU steps (sorted, moved state) the search still agrees with
+    the oracle applied to the GPU's own state."""
+    say("probe c0 ref after 10 steps")
+    p, parts = c0_ref.params[0], c0_ref.particles[0]
+    with pbf.Simulation(p, parts) as s:
+        s.step(10)
+        pos, vel = s.dump()
+        g = s.probe()
+        ncell = s.num_cells()
+    vel[:, 3] = 1.0
+    o = orc.probe_f32(p, pos, vel, ncell)
+    for k in ("cell_of", "sorted_ids", "cell_start", "nbr_count"):
+        assert np.array_equal(g[k], o[k]), k
+    brute = orc.brute_neighbors(o["pred"], float(p.x_max / p.x_n) ** 2)
+    assert brute.sum() > 0
+
+
+# ----------------------------------------------------------------------------
+# trajectories
+# ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nsteps", [1, 10])
+def test_ref_trajectory_c0(c0_ref, nsteps):
+    say("ref trajectory c0", nsteps)
+    p, parts = c0_ref.params[0], c0_ref.particles[0]
+    gp, gv = run_gpu(p, parts, nsteps)
+    pos, vel = orc.state_from_particles(parts)
+    op, ov = orc.run_f32(p, pos, vel, nsteps)
+    assert rel_err(gp[:, :3], op[:, :3]) <= 1e-5
+    assert rel_err(gv[:, :3], ov[:, :3]) <= 1e-5
+    assert np.array_equal(gv[:, 3], ov[:, 3])          # mass carried in .w
+    assert rel_err(gp[:, 3], op[:, 3]) <= 1e-5          # |v| in pos.w
+
+
+def test_ref_two_particles(golden_dir):
+    say("ref two.in")
+    p, _ = pbf.read_par(os.path.join(golden_dir, "two.par"))
+    parts = pbf.read_particles(os.path.join(golden_dir, "two.in"))
+    for nsteps in (1, 10):
+        gp, gv = run_gpu(p, parts, nsteps)
+        pos, vel = orc.state_from_particles(parts)
+        op, ov = orc.run_f32(p, pos, vel, nsteps)
+        assert rel_err(gp[:, :3], op[:, :3]) <= 1e-6
+        assert rel_err(gv[:, :3], ov[:, :3]) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["27", "box_small", "dam"])
+def test_ref_reference_scenarios(golden_dir, name):
+    say("ref scenario", name)
+    p, _ = pbf.read_par(os.path.join(golden_dir, name + ".par"))
+    parts = pbf.read_particles(os.path.join(golden_dir, name + ".in"))
+    gp, gv = run_gpu(p, parts, 10)
+    pos, vel = orc.state_from_particles(parts)
+    op, ov = orc.run_f32(p, pos, vel, 10)
+    assert rel_err(gp[:, :3], op[:, :3]) <= 1e-5
+    assert rel_err(gv[:, :3], ov[:, :3]) <= 1e-5
+
+
+def test_ref_wave_generator(c0_ref):
+    say("ref wave generator")
+    p, parts = c0_ref.params[0], c0_ref.particles[0]
+    gp, gv = run_gpu(p, parts, 5, wave=0.3)
+    pos, vel = orc.state_from_particles(parts)
+    op, ov = orc.run_f32(p, pos, vel, 5, wave=0.3)
+    assert rel_err(gp[:, :3], op[:, :3]) <= 1e-5
+    assert gp[:, 0].min() >= np.float32(0.3) + np.float32(0.1) - 1e-6
+
+
+def test_paper_vs_fp64_oracle():
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=21)
+    p, parts = sc.params[0], sc.particles[0]
+    say("paper vs fp64", sc.n)
+    gp, gv = run_gpu(p, parts, 10)
+    pos, vel = orc.state_from_particles(parts, np.float64)
+    op, ov = orc.run_f64(p, pos, vel, 10)
+    assert np.abs(gp[:, :3] - op[:, :3]).max() <= 1e-4
+    assert np.abs(gv[:, :3] - ov[:, :3]).max() <= 1e-3 * np.abs(ov[:, :3]).max()
+
+
+def test_paper_sphere_obstacle():
+    sc = scenarios.small_dam(dims=(12, 12, 12), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=4,
+                             sphere=(0.45, 0.3, 0.45, 0.15))
+    p, parts = sc.params[0], sc.particles[0]
+    say("paper sphere", sc.n)
+    gp, _ = run_gpu(p, parts, 10)
+    pos, vel = orc.state_from_particles(parts, np.float64)
+    op, _ = orc.run_f64(p, pos, vel, 10)
+    assert np.abs(gp[:, :3] - op[:, :3]).max() <= 1e-4
+    d = np.linalg.norm(gp[:, :3] - np.float32([0.45, 0.3, 0.45]), axis=1)
+    assert d.min() >= 0.15 + 0.1 - 1e-5
+
+
+# ----------------------------------------------------------------------------
+# edge cases
+# ----------------------------------------------------------------------------
+
+def test_empty_and_single_particle():
+    say("edge: empty and single")
+    p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF)
+    with pbf.Simulation(p, np.zeros((0, 10), np.float32)) as s:
+        s.step(3)
+        pos, vel = s.dump()
+        assert pos.shape == (0, 4)
+    one = np.zeros((1, 10), np.float32)
+    one[0, :4] = [1.0, 0.5, 0.5, 0.5]
+    gp, gv = run_gpu(p, one, 60)
+    pos, vel = orc.state_from_particles(one)
+    op, ov = orc.run_f32(p, pos, vel, 60)
+    assert np.array_equal(gp[:, :3], op[:, :3])
+    assert gp[0, 1] == np.float32(0.1)      # resting on the floor band
+
+
+def test_ragged_counts_and_outside_box():
+    """N not a multiple of the block; some particles start outside the box
+    (cell coordinates clamp into the grid; the wall clamp pulls them in)."""
+    rng = np.random.default_rng(5)
+    for n in (1, 63, 257, 1000):
+        say("edge: ragged", n)
+        p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF)
+        parts = np.zeros((n, 10), np.float32)
+        parts[:, 0] = 1.0
+        parts[:, 1:4] = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+        gp, gv = run_gpu(p, parts, 3)
+        pos, vel = orc.state_from_particles(parts)
+        op, ov = orc.run_f32(p, pos, vel, 3)
+        assert rel_err(gp[:, :3], op[:, :3]) <= 1e-5
+        assert gp[:, :3].min() >= np.float32(0.1) - 1e-6
+
+
+def test_all_particles_in_one_cell():
+    say("edge: one cell")
+    p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF)
+    rng = np.random.default_rng(9)
+    parts = np.zeros((200, 10), np.float32)
+    parts[:, 0] = 1.0
+    parts[:, 1:4] = (0.5 + rng.uniform(0.0, 0.09, (200, 3))).astype(np.float32)
+    with pbf.Simulation(p, parts) as s:
+        g = s.probe()
+    pos, vel = orc.state_from_particles(parts)
+    o = orc.probe_f32(p, pos, vel, 1000)
+    assert len(np.unique(o["cell_of"])) <= 8
+    for k in ("cell_of", "sorted_ids", "cell_start", "nbr_count"):
+        assert np.array_equal(g[k], o[k]), k
+
+
+# ----------------------------------------------------------------------------
+# batching, graphs, determinism, full size
+# ----------------------------------------------------------------------------
+
+def test_batch_matches_single_scenes():
+    c = scenarios.batch_config(nscenes=5, nmin=300, nmax=3000, seed=77)
+    say("batch vs single", c.meta["counts"])
+    with pbf.BatchSimulation(c.params, c.particles) as b:
+        assert b.num_scenes() == 5
+        b.step(6)
+        got = [b.dump_scene(k) for k in range(5)]
+    for k in range(5):
+        gp, gv = run_gpu(c.params[k], c.particles[k], 6)
+        assert np.array_equal(got[k][0], gp), k
+        assert np.array_equal(got[k][1], gv), k
+
+
+def test_graph_replays_match_eager_launches():
+    sc = scenarios.small_dam(dims=(20, 20, 20), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=8)
+    say("graph vs eager", sc.n)
+    a = pbf.Simulation(sc.params[0], sc.particles[0])
+    b = pbf.Simulation(sc.params[0], sc.particles[0])
+    try:
+        for _ in range(4):
+            a.step(1)                  # graph launch
+            b.profile(1)               # eager launches with events
+            pa, va = a.dump()
+            pb, vb = b.dump()
+            assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    finally:
+        a.close()
+        b.close()
+
+
+def test_full_size_c1_properties():
+    c1 = scenarios.config(1)
+    say("c1 full size", c1.n)
+    p, parts = c1.params[0], c1.particles[0]
+    with pbf.Simulation(p, parts) as s:
+        s.step(5)
+        s.synchronize()
+        pos, vel = s.dump()
+        d = s.diagnostics()
+        g = s.probe()
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    assert d["count"] == c1.n and np.isfinite(d["max_compression"])
+    # the sort is a permutation, grouped by non-decreasing cell
+    ids = g["sorted_ids"]
+    assert np.array_equal(np.sort(ids), np.arange(c1.n))
+    assert np.all(np.diff(g["cell_of"][ids]) >= 0)
+    cs = g["cell_start"]
+    assert cs[0] == 0 and cs[-1] == c1.n and np.all(np.diff(cs) >= 0)
+    # determinism: a second run gives identical bits
+    with pbf.Simulation(p, parts) as s2:
+        s2.step(5)
+        pos2, vel2 = s2.dump()
+    assert np.array_equal(pos, pos2) and np.array_equal(vel, vel2)
+    # momentum: the x and z components stay small next to the fall in y
+    assert abs(d["momentum"][1]) > 0
+
+
+def test_profile_and_timing_hooks():
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER)
+    say("profile hooks")
+    with pbf.Simulation(sc.params[0], sc.particles[0]) as s:
+        prof = s.profile(2)
+        assert prof["lambda"][1] == 8 and prof["delta"][1] == 8
+        assert all(ms >= 0 for ms, _ in prof.values())
+        assert s.time_steps(3) > 0
+        assert s.config()[0] in (64, 128, 256)

@@ -1,0 +1,154 @@
+"""CPU-side checks of the product library: it loads, exports every entry
+point include/pbf.h declares, validates parameters, and its scenario readers
+behave as the reference's ConfigReader / PartReader (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pbf
+import scenarios
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "pbf.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pbf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pbf.load_library()
+    names = header_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    # the Python binding declares the same set
+    assert sorted(pbf.EXPORTED) == names
+
+
+def test_params_init_defaults():
+    p = pbf.default_params(pbf.PHYSICS_REF)
+    assert p.iterations == 4 and p.relaxation == 1e4
+    assert p.gravity_y == np.float32(-9.81) and p.xsph_c == np.float32(0.01)
+    assert p.scorr_k == 0 and p.vorticity_eps == 0
+    q = pbf.default_params(pbf.PHYSICS_PAPER)
+    assert q.relaxation == 600 and q.scorr_n == 4 and q.scorr_dq == np.float32(0.2)
+    assert q.vorticity_eps == np.float32(1e-4)
+    with pytest.raises(pbf.PBFError):
+        pbf.default_params(7)
+
+
+def test_read_par_matches_reference_file(golden_dir):
+    p, part = pbf.read_par(os.path.join(golden_dir, "two.par"))
+    assert part == "two.in"
+    assert p.time_step_length == np.float32(0.005)
+    assert (p.x_min, p.x_max, p.x_n) == (0.0, 1.0, 100.0)
+    assert p.rest_density == np.float32(998200.0)
+    assert p.physics == pbf.PHYSICS_REF and p.iterations == 4
+    p2, part2 = pbf.read_par(os.path.join(golden_dir, "dam_coarse.par"))
+    assert part2 == "dam_coarse.in"
+    assert (p2.x_max, p2.x_n, p2.rest_density) == (30.0, 30.0, np.float32(9.261))
+
+
+def test_read_par_physics_keys_and_unknown_key(tmp_path):
+    f = tmp_path / "s.par"
+    f.write_text("part_input_file a.in\nx_max 2\nx_n 20\nphysics paper\n"
+                 "iterations 3\nscorr_k 0.002\nsphere 1 2 3 0.5\nbogus 1\n")
+    p, part = pbf.read_par(str(f))
+    assert part == "a.in" and p.physics == pbf.PHYSICS_PAPER
+    assert p.iterations == 3 and p.scorr_k == np.float32(0.002)
+    assert list(p.sphere) == [1.0, 2.0, 3.0, 0.5]
+    assert p.x_max == 2.0 and p.x_n == 20.0
+
+
+def test_read_par_missing_file():
+    with pytest.raises(pbf.PBFError):
+        pbf.read_par("/nonexistent/file.par")
+
+
+def test_read_particles(golden_dir):
+    a = pbf.read_particles(os.path.join(golden_dir, "two.in"))
+    assert a.shape == (2, 10)
+    assert np.array_equal(a[1, :4], np.float32([1, 0.49577011, 0.5, 0.5]))
+    b = pbf.read_particles(os.path.join(golden_dir, "27.in"))
+    assert b.shape[0] == 27
+    c = pbf.read_particles(os.path.join(golden_dir, "dam.in"))
+    ref = np.loadtxt(os.path.join(golden_dir, "dam.in"), skiprows=1)
+    assert c.shape[0] == ref.shape[0] == 1000
+    assert np.array_equal(c[:, :7], ref.astype(np.float32))
+
+
+def test_read_particles_stops_at_bad_mass(tmp_path):
+    f = tmp_path / "bad.in"
+    f.write_text("4\n1 0 0 0 0 0 0\n1 1 1 1 0 0 0\nxyz 1 1 1\n1 2 2 2 0 0 0\n")
+    a = pbf.read_particles(str(f))
+    assert a.shape[0] == 2  # PartReader.cpp:56-60: break on a bad mass
+
+
+def test_read_particles_short_file(tmp_path):
+    f = tmp_path / "short.in"
+    f.write_text("5\n1 0 0 0 0 0 0\n")
+    assert pbf.read_particles(str(f)).shape[0] == 1
+
+
+def test_create_rejects_bad_params():
+    p = pbf.default_params(pbf.PHYSICS_REF)
+    p.x_max = p.y_max = p.z_max = 1.0
+    p.x_n = p.y_n = p.z_n = 10.0
+    p.time_step_length, p.rest_density = 0.01, 1000.0
+    parts = np.zeros((1, 10), np.float32)
+    bad = p.copy()
+    bad.x_n = 10.5
+    with pytest.raises(pbf.PBFError) as e:
+        pbf.Simulation(bad, parts)
+    assert e.value.code == pbf.ERR_INVALID
+    bad = p.copy()
+    bad.iterations = 0
+    with pytest.raises(pbf.PBFError):
+        pbf.Simulation(bad, parts)
+    bad = p.copy()
+    bad.x_max = bad.x_min
+    with pytest.raises(pbf.PBFError):
+        pbf.Simulation(bad, parts)
+
+
+def test_scenario_configs_shapes():
+    c0 = scenarios.config(0)
+    assert c0.n == 32768 and c0.steps == 100
+    p = c0.params[0]
+    assert (p.x_max, p.y_max, p.z_max) == (4.0, 2.0, 2.0)
+    assert (p.x_n, p.y_n, p.z_n) == (40.0, 20.0, 20.0)
+    x = c0.particles[0][:, 1:4]
+    assert x.min() >= 0.1 and x[:, 0].max() < 4.0 - 0.1 and x[:, 1].max() < 2.0 - 0.1
+    # deterministic
+    assert np.array_equal(scenarios.config(0).particles[0], c0.particles[0])
+    # jitter bounded by 0.01 h
+    k = np.arange(32)
+    site = scenarios.MARGIN + k * scenarios.SPACING
+    assert np.abs(x[:32, 2] - site).max() <= 0.001 + 1e-7
+
+
+def test_batch_config_statistics():
+    c3 = scenarios.batch_config()
+    counts = np.array(c3.meta["counts"])
+    assert len(counts) == 256
+    assert counts.min() >= 2048 and counts.max() <= 65536
+    assert 3.0e6 < counts.sum() < 7.0e6
+    for p, parts in zip(c3.params, c3.particles):
+        box = np.array([p.x_max, p.y_max, p.z_max])
+        x = parts[:, 1:4]
+        assert x.min() > 0.1 - 0.002 and np.all(x.max(0) < box - 0.1 + 0.002)
+        cells = np.array([p.x_n, p.y_n, p.z_n])
+        assert np.allclose(box / cells, 0.1, rtol=1e-6)
+
+
+def test_splitmix64_known_values():
+    # splitmix64 of seed 0: first outputs (Steele, Lea & Flood 2014 reference
+    # implementation, as used by xoshiro seeding)
+    v = scenarios.splitmix64(0, 3)
+    assert [int(x) for x in v] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                   0x06C45D188009454F]

@@ -1,0 +1,328 @@
+"""Pins of the CPU oracle (oracle/pbf_oracle.cpp) against what the reference
+and the mathematics fix, so that the oracle can then check the HIP path.
+
+The reference ships no golden vectors or tests (its only fixtures are the
+input scenarios, copied under tests/golden/).  What pins the oracle here:
+  * the kernel constants against the -D strings src/main.cpp:149-168
+    produces for h = 0.1 (reproduced in the survey of the reference);
+  * perfect-lattice closed forms of rho and lambda in both physics modes;
+  * the two-particle scene of assets/scenarios/two.{par,in}, reduced to a
+    scalar recurrence on the separation and worked in FP64;
+  * brute-force O(N^2) neighbour counts, numpy cell ids and a numpy stable
+    sort against the oracle's linked-cell search;
+  * antisymmetry of the pressure term (sum of displacements = 0 away from
+    walls), mirror symmetry of a lattice, wall / wave-generator clamps,
+    the orientation of the vorticity of a rigid rotation, FP32 vs FP64.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import pbf
+import pbf_oracle as orc
+import scenarios
+
+
+def fmt6(v: float) -> np.float32:
+    """The float a '-DNAME=<%#g>f' flag of src/main.cpp:149-168 denotes."""
+    return np.float32(float(f"{v:#g}"))
+
+
+def params(box=(1.0, 1.0, 1.0), n=(10, 10, 10), physics=pbf.PHYSICS_REF, **kw):
+    p = pbf.Params()
+    p.physics = physics
+    p.iterations = 4
+    p.gravity_y = -9.81
+    p.xsph_c = 0.01
+    if physics == pbf.PHYSICS_REF:
+        p.relaxation = 1e4
+    else:
+        p.relaxation = 600.0
+        p.scorr_k, p.scorr_n, p.scorr_dq = 0.001, 4.0, 0.2
+        p.vorticity_eps = 1e-4
+    p.time_step_length = np.float32(1.0 / 120.0)
+    p.x_min = p.y_min = p.z_min = 0.0
+    p.x_max, p.y_max, p.z_max = box
+    p.x_n, p.y_n, p.z_n = (float(v) for v in n)
+    p.rest_density = 6378.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+# ----------------------------------------------------------------------------
+# constants
+# ----------------------------------------------------------------------------
+
+def test_ref_constants_match_reference_flags():
+    # box 2 x 1 x 1, 20 x 10 x 10 cells: the flags src/main.cpp:149-168 emits
+    p = params(box=(2.0, 1.0, 1.0), n=(20, 10, 10))
+    c = orc.constants(p)
+    expect = {"h": "0.100000", "h2": "0.0100000", "poly6": "1.56668e+09",
+              "spiky": "1.43239e+07", "dt": "0.00833333", "rho0": "6378.00",
+              "cl_x": "0.100000", "max_x": "2.00000", "n_x": "20.0000"}
+    for k, s in expect.items():
+        assert c[k] == np.float32(float(s)), (k, c[k], s)
+
+
+def test_paper_constants_exact():
+    p = params(box=(2.0, 1.0, 1.0), n=(20, 10, 10), physics=pbf.PHYSICS_PAPER)
+    c = orc.constants(p)
+    h = 2.0 / 20.0
+    assert c["poly6"] == np.float32(315.0 / (64.0 * math.pi * h ** 9))
+    assert c["spiky"] == np.float32(45.0 / (math.pi * h ** 6))
+    assert c["dt"] == np.float32(1.0 / 120.0)
+
+
+# ----------------------------------------------------------------------------
+# perfect lattice closed forms
+# ----------------------------------------------------------------------------
+
+# interior shells of a cubic lattice at spacing h/2 inside the support:
+# (count, q^2 / h^2); the 6 sites at q = h are excluded by r^2 < h^2
+SHELLS = [(6, 0.25), (12, 0.5), (8, 0.75)]
+
+
+def _lattice_state(h=0.125, n=9, origin=0.5):
+    s = h / 2
+    idx = np.arange(n)
+    g = np.stack(np.meshgrid(idx, idx, idx, indexing="ij"), -1).reshape(-1, 3)
+    pos = np.zeros((g.shape[0], 4), np.float32)
+    pos[:, :3] = origin + g * s   # exact binary fractions
+    vel = np.zeros_like(pos)
+    vel[:, 3] = 1.0
+    centre = int(np.flatnonzero((g == n // 2).all(1))[0])
+    return pos, vel, centre
+
+
+@pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
+def test_lattice_density_and_lambda_closed_form(physics):
+    h = 0.125
+    p = params(box=(2.0, 2.0, 2.0), n=(16, 16, 16), physics=physics, gravity_y=0.0)
+    pos, vel, ci = _lattice_state(h)
+    out = orc.probe_f32(p, pos, vel, 16 ** 3)
+    rho0 = 6378.0
+    if physics == pbf.PHYSICS_REF:
+        poly6 = float(fmt6(315.0 / (64.0 * math.pi * h ** 9)))
+        spiky = float(fmt6(45.0 / (math.pi * h ** 6)))
+        hh, h2 = float(fmt6(h)), float(fmt6(h * h))
+    else:
+        poly6 = 315.0 / (64.0 * math.pi * h ** 9)
+        spiky = 45.0 / (math.pi * h ** 6)
+        hh, h2 = h, h * h
+    rho = sum(c * poly6 * (h2 - f * h * h) ** 3 for c, f in SHELLS)
+    grad = [(c, spiky * (hh - math.sqrt(f) * h) ** 2) for c, f in SHELLS]
+    C = rho / rho0 - 1.0
+    if physics == pbf.PHYSICS_REF:
+        sum_abs = sum(c * g for c, g in grad)          # |sum grad W| = 0
+        lam = -C / (sum_abs ** 2 / rho0 ** 2 + 1e4)
+    else:
+        sum_sq = sum(c * g * g for c, g in grad)
+        lam = -C / (sum_sq / rho0 ** 2 + 600.0)
+    assert out["nbr_count"][ci] == 26
+    assert out["rho"][ci] == pytest.approx(rho, rel=2e-6)
+    assert out["lambda"][ci] == pytest.approx(lam, rel=2e-5)
+
+
+def test_survey_lattice_numbers_h01():
+    """h = 0.1: rho (self excluded) = 6511.52, lambda_paper = -2.507166e-5,
+    lambda_ref = -1.538864e-6 (closed forms quoted for this lattice)."""
+    h = 0.1
+    poly6 = 315.0 / (64.0 * math.pi * h ** 9)
+    rho = sum(c * poly6 * (h * h - f * h * h) ** 3 for c, f in SHELLS)
+    assert rho == pytest.approx(6511.5199, rel=1e-6)
+    # evaluate the oracle on such a lattice (binary-exact coordinates are not
+    # available at h = 0.1; the jitter-free lattice keeps q = h pairs just
+    # outside or inside the support, so use the 26-neighbour interior)
+    p = params(box=(4.0, 4.0, 4.0), n=(40, 40, 40), physics=pbf.PHYSICS_PAPER,
+               gravity_y=0.0)
+    pos, vel, ci = _lattice_state(h=0.1, n=9, origin=1.0)
+    out = orc.probe_f32(p, pos, vel, 40 ** 3)
+    if out["nbr_count"][ci] == 26:
+        assert out["rho"][ci] == pytest.approx(6511.5199, rel=1e-5)
+        assert out["lambda"][ci] == pytest.approx(-2.507166e-5, rel=1e-4)
+
+
+def test_lattice_centre_stays_put_fp64():
+    # rest density = the lattice's own density, so the block only relaxes
+    h = 0.125
+    poly6 = 315.0 / (64.0 * math.pi * h ** 9)
+    rho_lattice = sum(c * poly6 * (h * h - f * h * h) ** 3 for c, f in SHELLS)
+    p = params(box=(2.0, 2.0, 2.0), n=(16, 16, 16), physics=pbf.PHYSICS_PAPER,
+               gravity_y=0.0, rest_density=rho_lattice)
+    # a quarter spacing off the cell faces: the particles never change cell,
+    # so every candidate set is mirror-symmetric about the centre
+    pos, vel, ci = _lattice_state(h, origin=0.53125)
+    x, v = orc.run_f64(p, pos, vel, 3)
+    assert np.abs(x[ci, :3] - pos[ci, :3]).max() < 1e-12
+    assert np.abs(v[ci, :3]).max() < 1e-9
+    # the surface particles did move: the test is not vacuous
+    assert np.abs(x[:, :3] - pos[:, :3]).max() > 1e-6
+
+
+# ----------------------------------------------------------------------------
+# two particles (assets/scenarios/two.{par,in})
+# ----------------------------------------------------------------------------
+
+def _two_params():
+    p = params(box=(1.0, 1.0, 1.0), n=(100, 100, 100))
+    p.time_step_length = 0.005
+    p.rest_density = 998200.0
+    return p
+
+
+def test_two_particle_scene_scalar_recurrence(golden_dir):
+    parts = np.loadtxt(os.path.join(golden_dir, "two.in"), skiprows=1)
+    pos, vel = orc.state_from_particles(np.pad(parts, ((0, 0), (0, 3))), np.float64)
+    p = _two_params()
+    x, v = orc.run_f64(p, pos, vel, 1)
+
+    # the same step reduced to the separation d along x (ref physics), in
+    # the constants the kernels see
+    h = float(fmt6(0.01))
+    h2 = float(fmt6(0.01 ** 2))
+    poly6 = float(fmt6(315.0 / (64.0 * math.pi * float(np.float32(0.01)) ** 9)))
+    spiky = float(fmt6(45.0 / (math.pi * float(np.float32(0.01)) ** 6)))
+    dt = float(fmt6(np.float32(0.005)))
+    rho0 = float(fmt6(998200.0))
+    d = float(pos[0, 0] - pos[1, 0])
+    vy = float(np.float32(-9.81)) * dt   # the kernel's -9.81f literal
+    rho = 0.0
+    for _ in range(4):
+        rho = poly6 * (h2 - d * d) ** 3
+        g = spiky * (h - d) ** 2
+        lam = -(rho / rho0 - 1.0) / ((2 * g) ** 2 / rho0 ** 2 + 1e4)
+        step = 2 * lam * g / rho0      # |dp| of each particle, toward the other
+        d -= 2 * step
+    # both particles: equal and opposite x displacement, common fall in y
+    mid = 0.5 * (pos[0, 0] + pos[1, 0])
+    assert x[0, 0] == pytest.approx(mid + d / 2, abs=1e-13)
+    assert x[1, 0] == pytest.approx(mid - d / 2, abs=1e-13)
+    assert x[0, 1] == pytest.approx(0.5 + dt * vy, abs=1e-13)
+    # velocity (x* - x)/dt, then XSPH with the last lambda pass's density
+    vx0 = (x[0, 0] - pos[0, 0]) / dt
+    w = poly6 * (h2 - d * d) ** 3
+    dvx = float(np.float32(0.01)) * (1.0 / rho) * (-vx0 - vx0) * w
+    assert v[0, 0] == pytest.approx(vx0 + dvx, rel=1e-11)
+    assert v[1, 0] == pytest.approx(-(vx0 + dvx), rel=1e-11)
+    assert v[0, 1] == pytest.approx(vy, rel=1e-12)
+    assert x[0, 3] == pytest.approx(math.hypot(v[0, 0], v[0, 1]), rel=1e-12)
+    # rho < rho0: the bilateral constraint pulls the pair together
+    assert d < pos[0, 0] - pos[1, 0]
+
+
+# ----------------------------------------------------------------------------
+# neighbour search: cell ids, stable sort, stencil counts
+# ----------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def jittered():
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_REF, seed=11)
+    pos, vel = orc.state_from_particles(sc.particles[0])
+    return sc.params[0], pos, vel
+
+
+def test_cell_ids_sort_and_table(jittered):
+    p, pos, vel = jittered
+    out = orc.probe_f32(p, pos, vel, 20 ** 3)
+    pred = out["pred"]
+    # predict in float32, as predict_positions.cl:8-9
+    dt = fmt6(p.time_step_length)
+    vy = np.float32(vel[:, 1] + dt * np.float32(-9.81))
+    assert np.array_equal(pred[:, 1], np.float32(pos[:, 1] + dt * vy))
+    cl = fmt6(0.1)
+    cell = np.zeros(len(pos), np.int64)
+    stride = 1
+    for a in range(3):
+        c = np.trunc((pred[:, a] - np.float32(0.0)) / cl).astype(np.int64)
+        cell += np.clip(c, 0, 19) * stride
+        stride *= 20
+    assert np.array_equal(out["cell_of"], cell)
+    order = np.argsort(cell, kind="stable")
+    assert np.array_equal(out["sorted_ids"], order)
+    starts = np.searchsorted(cell[order], np.arange(20 ** 3 + 1), side="left")
+    assert np.array_equal(out["cell_start"], starts)
+
+
+def test_stencil_counts_equal_brute_force(jittered):
+    p, pos, vel = jittered
+    out = orc.probe_f32(p, pos, vel, 20 ** 3)
+    brute = orc.brute_neighbors(out["pred"], float(fmt6(0.01)))
+    assert np.array_equal(out["nbr_count"], brute)
+    assert brute.mean() > 20  # a dense block, not a vacuous comparison
+
+
+# ----------------------------------------------------------------------------
+# invariants of the step
+# ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
+def test_pressure_term_is_antisymmetric(physics):
+    sc = scenarios.small_dam(dims=(10, 10, 10), box=(2.0, 2.0, 2.0),
+                             physics=physics, seed=5)
+    p = sc.params[0]
+    p.gravity_y = 0.0
+    pos, vel = orc.state_from_particles(sc.particles[0], np.float64)
+    pos[:, :3] += 0.5  # away from every wall
+    x, _ = orc.run_f64(p, pos, vel, 1)
+    d = x[:, :3] - pos[:, :3]
+    assert np.abs(d).sum() > 1e-4
+    assert np.abs(d.sum(0)).max() < 1e-12 * max(1.0, np.abs(d).sum())
+
+
+def test_wall_and_wave_clamp():
+    p = params(box=(1.0, 1.0, 1.0), n=(10, 10, 10), gravity_y=0.0)
+    parts = np.zeros((3, 10), np.float32)
+    parts[:, 0] = 1.0
+    parts[0, 1:4] = [0.5, 0.02, 0.5]   # below the floor band
+    parts[1, 1:4] = [0.97, 0.5, 0.5]   # beyond x_max - h
+    parts[2, 1:4] = [0.25, 0.5, 0.5]   # inside x_min + wave + h for wave 0.2
+    pos, vel = orc.state_from_particles(parts)
+    x, _ = orc.run_f32(p, pos, vel, 1, wave=0.2)
+    h = fmt6(0.1)
+    assert x[0, 1] == np.float32(np.float32(0.0) + h)
+    assert x[1, 0] == np.float32(np.float32(1.0) - h)
+    assert x[2, 0] == np.float32(np.float32(np.float32(0.0) + np.float32(0.2)) + h)
+
+
+def test_vorticity_of_rigid_rotation_points_along_axis():
+    p = params(box=(2.0, 2.0, 2.0), n=(16, 16, 16), physics=pbf.PHYSICS_PAPER)
+    pos, _, ci = _lattice_state()
+    c = pos[ci, :3].astype(np.float64)
+    omega = np.array([0.3, -0.2, 1.0])
+    vel = np.zeros((pos.shape[0], 4))
+    vel[:, :3] = np.cross(omega, pos[:, :3] - c)
+    w = orc.vorticity_f64(p, pos.astype(np.float64), vel)[ci, :3]
+    cosang = w @ omega / (np.linalg.norm(w) * np.linalg.norm(omega))
+    assert cosang > 1 - 1e-9
+
+
+@pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
+def test_fp32_tracks_fp64(physics):
+    sc = scenarios.small_dam(dims=(10, 10, 10), box=(1.0, 1.0, 1.0),
+                             physics=physics, seed=3)
+    p = sc.params[0]
+    pos, vel = orc.state_from_particles(sc.particles[0])
+    x32, v32 = orc.run_f32(p, pos, vel, 10)
+    x64, v64 = orc.run_f64(p, pos, vel, 10)
+    assert np.abs(x32[:, :3] - x64[:, :3]).max() < 2e-5
+    assert np.abs(v32[:, :3] - v64[:, :3]).max() < 5e-3 * max(1.0, np.abs(v64).max())
+
+
+def test_literal_scorr_k01_diverges():
+    """s_corr k = 0.1 (with eps 600, rho0 6378, h 0.1, m 1) pushes the
+    surface layer ~0.03 m per iteration and diverges; k = 0.001 does not.
+    This is why PAPER physics defaults to k = 0.001."""
+    sc = scenarios.small_dam(dims=(12, 12, 12), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=3)
+    p = sc.params[0]
+    pos, vel = orc.state_from_particles(sc.particles[0], np.float64)
+    p.scorr_k = 0.1
+    _, v = orc.run_f64(p, pos, vel, 5)
+    assert not np.all(np.abs(v[:, :3]) < 100.0)
+    p.scorr_k = 0.001
+    _, v = orc.run_f64(p, pos, vel, 5)
+    assert np.all(np.abs(v[:, :3]) < 5.0)
