@@ -68,6 +68,72 @@ def cpu_baseline(params, parts, steps: int, threads: int):
     return parts.shape[0] * steps / dt, dt
 
 
+def _short(name: str) -> str:
+    name = name.replace("void ", "").replace("pbf::", "")
+    return name.split("(")[0]
+
+
+def rocprof_summary(root: str, particles: int, copy_mib: int) -> str:
+    """Markdown summary of the rocprofv3 databases written by the profiling
+    pass (sub-directories kt, fetch, write, valu, cfetch, cwrite)."""
+    import glob
+    import sqlite3
+
+    def db(sub):
+        files = glob.glob(os.path.join(root, sub, "*.db"))
+        return sqlite3.connect(files[0]) if files else None
+
+    out = []
+    kt = db("kt")
+    times = {}
+    if kt:
+        for name, calls, avg, tot in kt.execute(
+                "select name, count(*), avg(duration), sum(duration) from kernels "
+                "group by name order by sum(duration) desc"):
+            times[_short(name)] = (calls, avg / 1e3, tot / 1e6)
+
+    def counter(con, cname):
+        res = {}
+        if con is None:
+            return res
+        for name, avg in con.execute(
+                "select kernel_name, avg(v) from (select kernel_name, dispatch_id, "
+                "sum(value) as v from counters_collection where counter_name = ? "
+                "group by dispatch_id) group by kernel_name", (cname,)):
+            res[_short(name)] = avg
+        return res
+
+    fetch = counter(db("fetch"), "FETCH_SIZE")
+    write = counter(db("write"), "WRITE_SIZE")
+    cfetch = counter(db("cfetch"), "FETCH_SIZE").get("k_copy")
+    cwrite = counter(db("cwrite"), "WRITE_SIZE").get("k_copy")
+    copy_kib = copy_mib * 1024.0
+    fscale = copy_kib / cfetch if cfetch else None
+    wscale = copy_kib / cwrite if cwrite else None
+    valu_db = db("valu")
+    valu = counter(valu_db, "SQ_INSTS_VALU")
+    vmem = counter(valu_db, "SQ_INSTS_VMEM_RD")
+    waves = counter(valu_db, "SQ_WAVES")
+
+    out.append(f"copy calibration ({copy_mib} MiB read + write): FETCH_SIZE "
+               f"{cfetch} KiB, WRITE_SIZE {cwrite} KiB -> scale factors "
+               f"fetch {fscale}, write {wscale}")
+    out.append("")
+    out.append("| kernel | calls | avg us | total ms | HBM read B/particle | "
+               "HBM write B/particle | VALU instr/particle | VMEM rd instr/particle |")
+    out.append("|---|---|---|---|---|---|---|---|")
+    for k, (calls, avg_us, tot_ms) in times.items():
+        def per(v, s=1.0):
+            return "-" if v is None or s is None else f"{v * 1024.0 * s / particles:.1f}"
+        vi = valu.get(k)
+        vm = vmem.get(k)
+        out.append(f"| {k} | {calls} | {avg_us:.2f} | {tot_ms:.3f} | "
+                   f"{per(fetch.get(k), fscale)} | {per(write.get(k), wscale)} | "
+                   f"{'-' if vi is None else f'{vi * 64 / particles:.0f}'} | "
+                   f"{'-' if vm is None else f'{vm * 64 / particles:.0f}'} |")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,14 +143,29 @@ def main():
     ap.add_argument("--physics", choices=["paper", "ref"], default="paper")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain float4 copy of MIB MiB (counter "
+                         "calibration) and print its time")
+    ap.add_argument("--rocprof-summary", metavar="DIR",
+                    help="summarise the rocprofv3 databases under DIR and exit")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        n = scenarios.config(args.config).n if args.config != 3 else 0
+        print(rocprof_summary(args.rocprof_summary, n, 512))
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        ms = pbf.calibrate_copy(nbytes, 10)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms,
+                          "GBps": 2 * nbytes / (ms / 1e3) / 1e9}), flush=True)
+        return
 
     physics = pbf.PHYSICS_PAPER if args.physics == "paper" else pbf.PHYSICS_REF
     sc = scenarios.config(args.config, physics=physics)

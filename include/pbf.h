@@ -208,6 +208,12 @@ int pbf_profile_steps(pbf_sim *sim, uint32_t nsteps, double *ms,
  * (after the caller's warm-up); returns elapsed milliseconds in *ms. */
 int pbf_time_steps(pbf_sim *sim, uint32_t nsteps, double *ms);
 
+/* Measurement calibration: run a plain float4 copy of `bytes` bytes (read
+ * bytes, write bytes) `reps` times on the current device and return the
+ * average kernel time in *ms.  Its HBM counters give the scale factor for
+ * the per-kernel FETCH_SIZE / WRITE_SIZE readings. */
+int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms);
+
 /* Launch configuration chosen for this handle (after autotuning):
  * cfg[0] = gather block size, cfg[1] = particles per gather work-item
  * (1), cfg[2] = sort digit width in bits (0 = one-pass counting sort). */

@@ -704,6 +704,43 @@ int pbf_time_steps(pbf_sim *S, uint32_t nsteps, double *ms) {
   });
 }
 
+int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms) {
+  return guard([&] {
+    need(ms != nullptr && bytes >= 16 && reps >= 1, "bad argument");
+    const size_t n = (size_t)(bytes / sizeof(float4));
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (st) (void)hipStreamDestroy(st);
+    };
+    try {
+      PBF_HIP(hipMalloc(&a, n * sizeof(float4)));
+      PBF_HIP(hipMalloc(&b, n * sizeof(float4)));
+      PBF_HIP(hipMemset(a, 0, n * sizeof(float4)));
+      PBF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      PBF_HIP(hipEventCreate(&e0));
+      PBF_HIP(hipEventCreate(&e1));
+      pbf::enqueue_copy(a, b, n, st);  // warm-up
+      PBF_HIP(hipEventRecord(e0, st));
+      for (uint32_t r = 0; r < reps; ++r) pbf::enqueue_copy(a, b, n, st);
+      PBF_HIP(hipEventRecord(e1, st));
+      PBF_HIP(hipEventSynchronize(e1));
+      float t = 0.0f;
+      PBF_HIP(hipEventElapsedTime(&t, e0, e1));
+      *ms = t / reps;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
 int pbf_get_config(const pbf_sim *S, int32_t cfg[4]) {
   return guard([&] {
     need(S && cfg, "NULL argument");

@@ -96,6 +96,36 @@ __device__ __forceinline__ void for_stencil(const Scene &sc,
   }
 }
 
+// REF traversal: the reference's exact visiting order, so that every sum is
+// accumulated in the same order as the reference kernels: cells x-outer,
+// z-inner (compute_scaling.cl:33-53), and within a cell descending particle
+// id, the order of a list built by serial atomic_xchg insertion
+// (update_cells.cl:18).  Slots within a cell are in ascending id, so each
+// cell range is walked backwards.
+template <typename F>
+__device__ __forceinline__ void for_stencil_ref(const Scene &sc,
+                                                const int *__restrict__ cs,
+                                                float x, float y, float z,
+                                                F &&f) {
+  int cx, cy, cz;
+  cell_coords<kRef>(sc, x, y, z, cx, cy, cz);
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int nx = cx + dx;
+    if (nx < 0 || nx >= sc.nc[0]) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int ny = cy + dy;
+      if (ny < 0 || ny >= sc.nc[1]) continue;
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int nz = cz + dz;
+        if (nz < 0 || nz >= sc.nc[2]) continue;
+        const int c = sc.cell_base + nx + (ny + nz * sc.nc[1]) * sc.nc[0];
+        const int b = cs[c] + sc.shift;
+        for (int j = cs[c + 1] + sc.shift - 1; j >= b; --j) f(j);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // REF pair terms, in the reference kernels' operation order, no contraction
 // ---------------------------------------------------------------------------
@@ -438,7 +468,7 @@ __global__ __launch_bounds__(256) void k_lambda(
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   if constexpr (PH == kRef) {
     float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
-    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
       if (j != i) ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
     });
     lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
@@ -494,7 +524,7 @@ __global__ __launch_bounds__(256) void k_delta(
   float3 dp;
   if constexpr (PH == kRef) {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
       if (j != i) ref_delta_pair(sc, xi, X[j], li, lam[j], sx, sy, sz);
     });
     dp = make_float3(sx / sc.rho0, sy / sc.rho0, sz / sc.rho0);
@@ -565,7 +595,7 @@ __global__ __launch_bounds__(256) void k_xsph_final(
   const float4 vi = V[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
   if constexpr (PH == kRef) {
-    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
       if (j != i) ref_xsph_pair(sc, xi, X[j], vi, V[j], sx, sy, sz);
     });
   } else {
@@ -932,6 +962,24 @@ const char *kernel_class_name(int cls) {
       "predict_count", "scan", "scatter", "reorder", "lambda",
       "delta",         "xsph", "vorticity"};
   return (cls >= 0 && cls < kNumClasses) ? names[cls] : "?";
+}
+
+}  // namespace pbf
+
+namespace pbf {
+
+// Plain streaming copy (measurement calibration only).
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a,
+                                              float4 *__restrict__ b,
+                                              size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    b[i] = a[i];
+}
+
+void enqueue_copy(const float4 *a, float4 *b, size_t n, hipStream_t st) {
+  k_copy<<<2048, 256, 0, st>>>(a, b, n);
 }
 
 }  // namespace pbf

@@ -51,5 +51,6 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st);
 void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   double *out_dev);
 const char *kernel_class_name(int cls);
+void enqueue_copy(const float4 *a, float4 *b, size_t n, hipStream_t st);
 
 }  // namespace pbf

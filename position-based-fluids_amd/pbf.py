@@ -128,6 +128,8 @@ _SIGNATURES = [
       _P(ctypes.c_char_p), _P(ctypes.c_uint32)]),
     ("pbf_time_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_double)]),
     ("pbf_get_config", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32)]),
+    ("pbf_calibrate_copy", ctypes.c_int,
+     [ctypes.c_uint64, ctypes.c_uint32, _P(ctypes.c_double)]),
     ("pbf_read_par", ctypes.c_int,
      [ctypes.c_char_p, _P(Params), ctypes.c_char_p, ctypes.c_size_t]),
     ("pbf_read_particles", ctypes.c_int,
@@ -351,6 +353,14 @@ class BatchSimulation(Simulation):
 
     def num_scenes(self) -> int:
         return int(self._lib.pbf_num_scenes(self._h))
+
+
+def calibrate_copy(nbytes: int, reps: int = 10) -> float:
+    """Average time (ms) of a plain float4 copy of nbytes (read + write)."""
+    lib = load_library()
+    ms = ctypes.c_double()
+    _check(lib.pbf_calibrate_copy(nbytes, reps, ctypes.byref(ms)))
+    return ms.value
 
 
 def read_par(path: str):

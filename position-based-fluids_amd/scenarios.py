@@ -220,9 +220,9 @@ def batch_config(nscenes: int = 256, physics: int = pbf.PHYSICS_PAPER,
 
 
 def small_dam(dims=(8, 8, 8), box=(1.0, 1.0, 1.0), physics=pbf.PHYSICS_REF,
-              seed: int = 7, **kw) -> Scenario:
-    """A small jittered dam block for tests."""
-    pos = lattice(dims, (MARGIN, MARGIN, MARGIN), seed=seed)
+              seed: int = 7, origin=(MARGIN, MARGIN, MARGIN), **kw) -> Scenario:
+    """A small jittered dam block for tests (first site at ``origin``)."""
+    pos = lattice(dims, origin, seed=seed)
     params = make_params(box, physics, **kw)
     return Scenario("small", [params], [particles_from_positions(pos)], 10,
                     {"dims": dims, "box": box})
