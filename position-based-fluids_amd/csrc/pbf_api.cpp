@@ -335,6 +335,8 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     d.vx = S.alloc<float4>(P);
     d.om = S.alloc<float4>(P);
   }
+  d.nl = S.alloc<uint4>(P * (size_t)(pbf::kMaxHits / 8));
+  d.ncnt = S.alloc<int>(P);
   d.nbr = S.alloc<int>(P);
   d.red = S.alloc<double>((P / 64 + 1) * 8);
   d.err = S.alloc<int>(1);

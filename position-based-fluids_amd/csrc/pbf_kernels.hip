@@ -151,14 +151,15 @@ __device__ __forceinline__ float dist2_exact(float3 xi, float4 xj, float &rx,
 }
 
 // compute_scaling.cl:56-84 for one list entry.
-__device__ __forceinline__ void ref_lambda_pair(const Scene &sc, float3 xi,
+__device__ __forceinline__ bool ref_lambda_pair(const Scene &sc, float3 xi,
                                                 float4 xj, float &dens,
                                                 float &gsum, float &gix,
                                                 float &giy, float &giz) {
   PBF_REF_EXACT
   float rx, ry, rz;
   const float r2 = dist2_exact(xi, xj, rx, ry, rz);
-  if (r2 > 0.0f && r2 < sc.h2) {
+  const bool hit = r2 > 0.0f && r2 < sc.h2;
+  if (hit) {
     const float rl = sqrtf(r2);
     const float gx = rx / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float gy = ry / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
@@ -170,6 +171,7 @@ __device__ __forceinline__ void ref_lambda_pair(const Scene &sc, float3 xi,
     giy += gy;
     giz += gz;
   }
+  return hit;
 }
 
 // compute_scaling.cl:129-140.
@@ -451,12 +453,154 @@ __global__ __launch_bounds__(256) void k_reorder(
 }
 
 // ---------------------------------------------------------------------------
-// constraint solve
+// hit lists (PAPER)
+//
+// The lambda pass of an iteration and the delta pass that follows it see the
+// same positions, so they have the same neighbours (0 < r^2 < h^2); so do the
+// XSPH pass and the vorticity pass.  The searching pass records its hits in
+// visiting order and the following pass replays them: the same sums in the
+// same order, without walking the 27 cells again.
+//
+// An entry is 16 bits: stencil row (dz, dy) in the top 4 bits, offset of the
+// slot from the start of that row's slot range in the low 12.  Entries are
+// packed eight to a 16-byte group in registers (a 128-bit shift chain) and
+// each full group is one store; group g of slot i is at nl[g * P + i], so
+// lanes store and load adjacent groups.  A particle with more than kMaxHits
+// hits (or a row longer than 4096 slots) is marked with a count of
+// kMaxHits + 1 and searched again by the replaying pass.
 // ---------------------------------------------------------------------------
 
-// compute_scaling.cl:12-140 (REF) / eqs. 1, 2, 8, 9, 11 (PAPER).
-template <int PH>
-__global__ __launch_bounds__(256) void k_lambda(
+struct HitRecorder {
+  unsigned w0, w1, w2, w3;
+  int c;
+  bool over;
+  __device__ __forceinline__ void push(unsigned e) {
+    w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+    w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+    w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+    w3 = __builtin_amdgcn_alignbit(e, w3, 16);
+  }
+  __device__ __forceinline__ void add(int j, int row, int b,
+                                     uint4 *__restrict__ nl, int stride,
+                                     int i) {
+    const int off = j - b;
+    if (c < kMaxHits && off < 4096) {
+      push((unsigned)((row << 12) | off));
+      if ((c & 7) == 7)
+        nl[(size_t)(c >> 3) * stride + i] = make_uint4(w0, w1, w2, w3);
+    } else {
+      over = true;
+    }
+    ++c;
+  }
+  // store the last partial group (aligned to its first entry) and the count
+  __device__ __forceinline__ void flush(uint4 *__restrict__ nl,
+                                       int *__restrict__ ncnt, int stride,
+                                       int i) {
+    if (over) {
+      ncnt[i] = kMaxHits + 1;
+      return;
+    }
+    ncnt[i] = c;
+    if (c & 7) {
+      for (int k = c & 7; k < 8; ++k) push(0u);
+      nl[(size_t)(c >> 3) * stride + i] = make_uint4(w0, w1, w2, w3);
+    }
+  }
+};
+
+// Stencil walk of a PAPER searching pass: f(j, X[j], row, row_start) for
+// every slot of the 9 row ranges, candidate positions loaded four at a time
+// so that four loads are in flight per wait.
+template <typename F>
+__device__ __forceinline__ void walk_rows(const Scene &sc,
+                                          const int *__restrict__ cs,
+                                          const float4 *__restrict__ X,
+                                          float3 xi, F &&f) {
+  int cx, cy, cz;
+  cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
+  const int xlo = max(cx - 1, 0), xhi = min(cx + 1, sc.nc[0] - 1);
+  if (xlo > xhi) return;
+  for (int dz = 0; dz < 3; ++dz) {
+    const int zz = cz - 1 + dz;
+    if (zz < 0 || zz >= sc.nc[2]) continue;
+    for (int dy = 0; dy < 3; ++dy) {
+      const int yy = cy - 1 + dy;
+      if (yy < 0 || yy >= sc.nc[1]) continue;
+      const int base = sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0];
+      const int b = cs[base + xlo] + sc.shift;
+      const int e = cs[base + xhi + 1] + sc.shift;
+      const int row = dz * 3 + dy;
+      int j = b;
+      for (; j + 4 <= e; j += 4) {
+        const float4 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2], a3 = X[j + 3];
+        f(j, a0, row, b);
+        f(j + 1, a1, row, b);
+        f(j + 2, a2, row, b);
+        f(j + 3, a3, row, b);
+      }
+      for (; j < e; ++j) f(j, X[j], row, b);
+    }
+  }
+}
+
+// Replay of the recorded hits of slot i: f(j, X[j], true) in recorded
+// order, eight entries resolved and their eight loads issued before any is
+// used.  An overflowed list walks the stencil instead: f(j, X[j], false),
+// the caller testing r^2.  rb is LDS scratch of 9 x blockDim.x ints.
+template <typename F>
+__device__ __forceinline__ void replay_hits(const Scene &sc,
+                                            const int *__restrict__ cs,
+                                            const float4 *__restrict__ X,
+                                            const uint4 *__restrict__ nl,
+                                            const int *__restrict__ ncnt,
+                                            int stride, int i, float3 xi,
+                                            int *rb, F &&f) {
+  const int cnt = ncnt[i];
+  if (cnt <= kMaxHits) {
+    const int lane = threadIdx.x, bs = blockDim.x;
+    int cx, cy, cz;
+    cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
+    const int xlo = max(cx - 1, 0);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
+      int start = 0;
+      if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1])
+        start = cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + xlo] +
+                sc.shift;
+      rb[r * bs + lane] = start;
+    }
+    for (int g = 0; g * 8 < cnt; ++g) {
+      const uint4 w = nl[(size_t)g * stride + i];
+      const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+      int js[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const unsigned e = (ws[q >> 1] >> ((q & 1) * 16)) & 0xffffu;
+        js[q] = (g * 8 + q < cnt) ? rb[(e >> 12) * bs + lane] + (int)(e & 0xfffu)
+                                  : i;
+      }
+      float4 xs[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) xs[q] = X[js[q]];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (g * 8 + q < cnt) f(js[q], xs[q], true);
+    }
+  } else {
+    walk_rows(sc, cs, X, xi,
+              [&](int j, float4 xj, int, int) { f(j, xj, false); });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// constraint solve, REF (reference arithmetic and order; lambda array,
+// x* ping-pong)
+// ---------------------------------------------------------------------------
+
+// compute_scaling.cl:12-140.
+__global__ __launch_bounds__(256) void k_lambda_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
     float *__restrict__ lam, float *__restrict__ rho) {
@@ -466,49 +610,19 @@ __global__ __launch_bounds__(256) void k_lambda(
   if (i - sc.slot_off >= sc.n) return;
   const float4 x4 = X[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  if constexpr (PH == kRef) {
-    float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
-    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      if (j != i) ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
-    });
-    lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
-    rho[i] = dens;
-  } else {
-    // sums of (h^2-r^2)^3, (h-r)^4 and (h-r)^2 r/|r|; constants applied once
-    float wsum = 0.0f, g2 = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    const float h = sc.h, h2 = sc.h2;
-    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      const float4 xj = X[j];
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float r2 = rx * rx + ry * ry + rz * rz;
-      if (r2 > 0.0f && r2 < h2) {
-        const float inv = __builtin_amdgcn_rsqf(r2);
-        const float hr = h - r2 * inv;
-        const float t = h2 - r2;
-        const float hr2 = hr * hr;
-        wsum += t * t * t;
-        g2 += hr2 * hr2;
-        const float gs = hr2 * inv;
-        gx += rx * gs;
-        gy += ry * gs;
-        gz += rz * gs;
-      }
-    });
-    const float dens = sc.poly6 * wsum;
-    const float sp2 = sc.spiky * sc.spiky;
-    const float denom =
-        sp2 * (gx * gx + gy * gy + gz * gz + g2) * (sc.inv_rho0 * sc.inv_rho0) +
-        sc.eps;
-    lam[i] = -(dens * sc.inv_rho0 - 1.0f) / denom;
-    rho[i] = dens;
-  }
+  float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
+  for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    if (j != i) ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
+  });
+  lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
+  rho[i] = dens;
 }
 
-// compute_delta.cl:13-152 + update_predicted.cl (REF) / eqs. 13, 14
-// (PAPER).  LAST also forms v = (x* - x)/dt (update_velocities.cl:8) and
-// carries rho of the last lambda pass in .w for the XSPH pass.
-template <int PH, bool LAST>
-__global__ __launch_bounds__(256) void k_delta(
+// compute_delta.cl:13-152 + update_predicted.cl.  LAST also forms
+// v = (x* - x)/dt (update_velocities.cl:8) and carries rho of the last
+// lambda pass in .w for the XSPH pass.
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_delta_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
     const float *__restrict__ lam, const float *__restrict__ rho,
@@ -521,40 +635,12 @@ __global__ __launch_bounds__(256) void k_delta(
   const float4 x4 = X[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float li = lam[i];
-  float3 dp;
-  if constexpr (PH == kRef) {
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      if (j != i) ref_delta_pair(sc, xi, X[j], li, lam[j], sx, sy, sz);
-    });
-    dp = make_float3(sx / sc.rho0, sy / sc.rho0, sz / sc.rho0);
-  } else {
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
-    const int n = sc.scorr_n;
-    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      const float4 xj = X[j];
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float r2 = rx * rx + ry * ry + rz * rz;
-      if (r2 > 0.0f && r2 < h2) {
-        const float inv = __builtin_amdgcn_rsqf(r2);
-        const float hr = h - r2 * inv;
-        const float t = h2 - r2;
-        const float q = t * t * t * cq;  // W / W(dq)
-        float qn = 1.0f;
-        for (int e = 0; e < n; ++e) qn *= q;
-        const float coef = li + lam[j] - kk * qn;
-        const float gs = coef * hr * hr * inv;
-        ax += rx * gs;
-        ay += ry * gs;
-        az += rz * gs;
-      }
-    });
-    // grad_i W = -spiky (h-r)^2 r/|r|
-    const float c = -sc.spiky * sc.inv_rho0;
-    dp = make_float3(ax * c, ay * c, az * c);
-  }
-  const float3 xn = clamp_update(sc, xi, dp, PH == kPaper && sc.sphere[3] > 0.0f);
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    if (j != i) ref_delta_pair(sc, xi, X[j], li, lam[j], sx, sy, sz);
+  });
+  const float3 dp = make_float3(sx / sc.rho0, sy / sc.rho0, sz / sc.rho0);
+  const float3 xn = clamp_update(sc, xi, dp, false);
   Xn[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
   if constexpr (LAST) {
     const float4 xo = xold[i];
@@ -577,10 +663,8 @@ __device__ __forceinline__ void write_final(int i, float3 xs, float4 v,
     atomicOr(err, 1);
 }
 
-// apply_vorticity_and_viscosity.cl:12-104 fused with update_positions.cl
-// (REF, and PAPER without vorticity).
-template <int PH>
-__global__ __launch_bounds__(256) void k_xsph_final(
+// apply_vorticity_and_viscosity.cl:12-104 fused with update_positions.cl.
+__global__ __launch_bounds__(256) void k_xsph_final_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
     const float4 *__restrict__ V, const int *__restrict__ pid_s,
@@ -594,58 +678,176 @@ __global__ __launch_bounds__(256) void k_xsph_final(
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 vi = V[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  if constexpr (PH == kRef) {
-    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      if (j != i) ref_xsph_pair(sc, xi, X[j], vi, V[j], sx, sy, sz);
-    });
-  } else {
-    const float h2 = sc.h2;
-    for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      const float4 xj = X[j];
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float r2 = rx * rx + ry * ry + rz * rz;
-      if (r2 > 0.0f && r2 < h2) {
-        const float4 vj = V[j];
-        const float t = h2 - r2;
-        // a neighbour with no density yet (entered the support after the
-        // last lambda pass) carries no weight
-        const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
-        sx += (vj.x - vi.x) * w;
-        sy += (vj.y - vi.y) * w;
-        sz += (vj.z - vi.z) * w;
-      }
-    });
-    const float c = sc.poly6;
-    sx *= c;
-    sy *= c;
-    sz *= c;
-  }
+  for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    if (j != i) ref_xsph_pair(sc, xi, X[j], vi, V[j], sx, sy, sz);
+  });
   const float4 dv = mul_exact(sc.c_xsph, sx, sy, sz);
   float len;
   const float4 v = add_len_exact(vi, dv.x, dv.y, dv.z, len);
   write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
 }
 
-// PAPER: XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15).
-__global__ __launch_bounds__(256) void k_xsph_omega(
+// ---------------------------------------------------------------------------
+// constraint solve, PAPER
+//
+// Buffers: A holds x* (after the last delta pass .w = rho); the lambda pass
+// writes B = (x*, lambda) so that the delta pass reads one float4 per
+// neighbour, and writes the new x* back into A.
+// ---------------------------------------------------------------------------
+
+// eqs. 1, 2, 8, 9, 11; records the hit list for the delta pass.
+__global__ __launch_bounds__(256) void k_lambda_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ X, const int *__restrict__ cs,
-    const float4 *__restrict__ V, float4 *__restrict__ VX,
-    float4 *__restrict__ OM) {
+    const float4 *__restrict__ A, const int *__restrict__ cs,
+    float4 *__restrict__ B, float *__restrict__ rho, uint4 *__restrict__ nl,
+    int *__restrict__ ncnt, int stride) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = X[i];
+  const float4 x4 = A[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
+  // sums of (h^2-r^2)^3, (h-r)^4 and (h-r)^2 r/|r|; constants applied once
+  float wsum = 0.0f, g2 = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  const float h = sc.h, h2 = sc.h2;
+  walk_rows(sc, cs, A, xi, [&](int j, float4 xj, int row, int b) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 > 0.0f && r2 < h2) {
+      rec.add(j, row, b, nl, stride, i);
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float t = h2 - r2;
+      const float hr2 = hr * hr;
+      wsum += t * t * t;
+      g2 += hr2 * hr2;
+      const float gs = hr2 * inv;
+      gx += rx * gs;
+      gy += ry * gs;
+      gz += rz * gs;
+    }
+  });
+  rec.flush(nl, ncnt, stride, i);
+  const float dens = sc.poly6 * wsum;
+  const float sp2 = sc.spiky * sc.spiky;
+  const float denom =
+      sp2 * (gx * gx + gy * gy + gz * gz + g2) * (sc.inv_rho0 * sc.inv_rho0) +
+      sc.eps;
+  B[i] = make_float4(xi.x, xi.y, xi.z, -(dens * sc.inv_rho0 - 1.0f) / denom);
+  rho[i] = dens;
+}
+
+// eqs. 13, 14 and the collision clamp over the hits of the lambda pass.
+// LAST also forms v = (x* - x)/dt and carries rho in .w for XSPH.
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_delta_paper(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ B, const int *__restrict__ cs,
+    const float *__restrict__ rho, const float4 *__restrict__ xold,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
+    float4 *__restrict__ A, float4 *__restrict__ V) {
+  extern __shared__ int rb_lds[];
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = B[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  const float li = x4.w;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
+  const int n = sc.scorr_n;
+  replay_hits(sc, cs, B, nl, ncnt, stride, i, xi, rb_lds,
+              [&](int, float4 xj, bool hit) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (hit || (r2 > 0.0f && r2 < h2)) {
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float t = h2 - r2;
+      const float q = t * t * t * cq;  // W / W(dq)
+      float qn = 1.0f;
+      for (int e = 0; e < n; ++e) qn *= q;
+      const float coef = li + xj.w - kk * qn;
+      const float gs = coef * hr * hr * inv;
+      ax += rx * gs;
+      ay += ry * gs;
+      az += rz * gs;
+    }
+  });
+  // grad_i W = -spiky (h-r)^2 r/|r|
+  const float c = -sc.spiky * sc.inv_rho0;
+  const float3 dp = make_float3(ax * c, ay * c, az * c);
+  const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
+  A[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
+  if constexpr (LAST) {
+    const float4 xo = xold[i];
+    const float3 v = velocity_exact(sc, xn, xo);
+    V[i] = make_float4(v.x, v.y, v.z, xo.w);
+  }
+}
+
+// XSPH with 1/rho_j weights fused with update_positions.cl (PAPER without
+// vorticity).
+__global__ __launch_bounds__(256) void k_xsph_final_paper(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ A, const int *__restrict__ cs,
+    const float4 *__restrict__ V, const int *__restrict__ pid_s,
+    float4 *__restrict__ pos, float4 *__restrict__ vel,
+    int *__restrict__ pid, int *__restrict__ err) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = A[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  const float4 vi = V[i];
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  const float h2 = sc.h2;
+  walk_rows(sc, cs, A, xi, [&](int j, float4 xj, int, int) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 > 0.0f && r2 < h2) {
+      const float4 vj = V[j];
+      const float t = h2 - r2;
+      // a neighbour with no density yet (entered the support after the
+      // last lambda pass) carries no weight
+      const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
+      sx += (vj.x - vi.x) * w;
+      sy += (vj.y - vi.y) * w;
+      sz += (vj.z - vi.z) * w;
+    }
+  });
+  const float c = sc.c_xsph * sc.poly6;
+  const float4 v = make_float4(vi.x + c * sx, vi.y + c * sy, vi.z + c * sz, vi.w);
+  const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+}
+
+// XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15);
+// writes B = (x*, |omega|) and records the hit list for the vorticity pass.
+__global__ __launch_bounds__(256) void k_xsph_omega(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ A, const int *__restrict__ cs,
+    const float4 *__restrict__ V, float4 *__restrict__ VX,
+    float4 *__restrict__ OM, float4 *__restrict__ B, uint4 *__restrict__ nl,
+    int *__restrict__ ncnt, int stride) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = A[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 vi = V[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
   const float h = sc.h, h2 = sc.h2;
-  for_stencil<kPaper>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-    const float4 xj = X[j];
+  HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
+  walk_rows(sc, cs, A, xi, [&](int j, float4 xj, int row, int b) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
+      rec.add(j, row, b, nl, stride, i);
       const float4 vj = V[j];
       const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
       const float t = h2 - r2;
@@ -662,39 +864,45 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
       oz += vx * gy - vy * gx;
     }
   });
+  rec.flush(nl, ncnt, stride, i);
   const float cx = sc.c_xsph * sc.poly6;
   VX[i] = make_float4(vi.x + cx * sx, vi.y + cx * sy, vi.z + cx * sz, vi.w);
   ox *= sc.spiky;
   oy *= sc.spiky;
   oz *= sc.spiky;
-  OM[i] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
+  const float wl = sqrtf(ox * ox + oy * oy + oz * oz);
+  OM[i] = make_float4(ox, oy, oz, wl);
+  B[i] = make_float4(xi.x, xi.y, xi.z, wl);
 }
 
-// PAPER: eta = sum (|w_j| - |w_i|) grad_i W_ij, N = eta/|eta|,
-// v += dt eps (N x w_i) (eq. 16), then update_positions.cl.
+// eta = sum (|w_j| - |w_i|) grad_i W_ij, N = eta/|eta|, v += dt eps
+// (N x w_i) (eq. 16), then update_positions.cl; replays the XSPH hits over
+// B = (x*, |omega|).
 __global__ __launch_bounds__(256) void k_vort_final(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ X, const int *__restrict__ cs,
+    const float4 *__restrict__ B, const int *__restrict__ cs,
     const float4 *__restrict__ VX, const float4 *__restrict__ OM,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
     const int *__restrict__ pid_s, float4 *__restrict__ pos,
     float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err) {
+  extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = X[i];
+  const float4 x4 = B[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 wi = OM[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   const float h = sc.h, h2 = sc.h2;
-  for_stencil<kPaper>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-    const float4 xj = X[j];
+  replay_hits(sc, cs, B, nl, ncnt, stride, i, xi, rb_lds,
+              [&](int, float4 xj, bool hit) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && r2 < h2) {
+    if (hit || (r2 > 0.0f && r2 < h2)) {
       const float inv = __builtin_amdgcn_rsqf(r2);
       const float hr = h - r2 * inv;
-      const float gs = (OM[j].w - wi.w) * hr * hr * inv;
+      const float gs = (xj.w - wi.w) * hr * hr * inv;
       ex += rx * gs;
       ey += ry * gs;
       ez += rz * gs;
@@ -718,6 +926,18 @@ __global__ __launch_bounds__(256) void k_vort_final(
 // ---------------------------------------------------------------------------
 // probes and diagnostics
 // ---------------------------------------------------------------------------
+
+// PAPER probe: lambda from B = (x*, lambda) into the lambda array.
+__global__ __launch_bounds__(256) void k_unpack_lambda(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ B, float *__restrict__ lam) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  lam[i] = B[i].w;
+}
+
 
 // Stencil neighbour count with the reference predicate 0 < r^2 < h^2,
 // r^2 unfused (compute_scaling.cl:56-62).
@@ -851,12 +1071,14 @@ struct Launcher {
 
 }  // namespace
 
-#define PBF_LAUNCH(L, cls, kern, grid, block, ...)                 \
+#define PBF_LAUNCH_SMEM(L, cls, kern, grid, block, smem, ...)      \
   do {                                                             \
     (L).begin(cls);                                                \
-    kern<<<(grid), (block), 0, (L).st>>>(__VA_ARGS__);             \
+    kern<<<(grid), (block), (smem), (L).st>>>(__VA_ARGS__);        \
     (L).end(cls);                                                  \
   } while (0)
+#define PBF_LAUNCH(L, cls, kern, grid, block, ...) \
+  PBF_LAUNCH_SMEM(L, cls, kern, grid, block, 0, __VA_ARGS__)
 
 template <int PH>
 static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
@@ -879,37 +1101,60 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
              d.xa, d.xold);
 }
 
-template <int PH>
-static void step_impl(const DevBuffers &d, Launcher &L) {
+static void step_ref(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
   const int bs = sh.block;
   const int grid = sh.slots / bs;
-  sort_stage<PH>(d, L, 1);
+  sort_stage<kRef>(d, L, 1);
   float4 *cur = d.xa, *nxt = d.xb;
   for (int it = 0; it < sh.iterations; ++it) {
-    const bool last = it == sh.iterations - 1;
-    PBF_LAUNCH(L, kClsLambda, k_lambda<PH>, grid, bs, d.scenes, d.block_scene,
+    PBF_LAUNCH(L, kClsLambda, k_lambda_ref, grid, bs, d.scenes, d.block_scene,
                cur, d.cell_start, d.lam, d.rho);
-    if (last) {
-      PBF_LAUNCH(L, kClsDelta, (k_delta<PH, true>), grid, bs, d.scenes,
+    if (it == sh.iterations - 1) {
+      PBF_LAUNCH(L, kClsDelta, k_delta_ref<true>, grid, bs, d.scenes,
                  d.block_scene, cur, d.cell_start, d.lam, d.rho, d.xold, nxt,
                  d.v);
     } else {
-      PBF_LAUNCH(L, kClsDelta, (k_delta<PH, false>), grid, bs, d.scenes,
+      PBF_LAUNCH(L, kClsDelta, k_delta_ref<false>, grid, bs, d.scenes,
                  d.block_scene, cur, d.cell_start, d.lam, d.rho, d.xold, nxt,
                  d.v);
     }
     std::swap(cur, nxt);
   }
-  if (PH == kPaper && sh.vorticity) {
+  PBF_LAUNCH(L, kClsXsph, k_xsph_final_ref, grid, bs, d.scenes, d.block_scene,
+             cur, d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err);
+}
+
+static void step_paper(const DevBuffers &d, Launcher &L) {
+  const StepShape &sh = L.sh;
+  const int bs = sh.block;
+  const int grid = sh.slots / bs;
+  sort_stage<kPaper>(d, L, 1);
+  const size_t rb_smem = (size_t)9 * bs * sizeof(int);  // replay row starts
+  for (int it = 0; it < sh.iterations; ++it) {
+    PBF_LAUNCH(L, kClsLambda, k_lambda_paper, grid, bs, d.scenes,
+               d.block_scene, d.xa, d.cell_start, d.xb, d.rho, d.nl, d.ncnt,
+               sh.slots);
+    if (it == sh.iterations - 1) {
+      PBF_LAUNCH_SMEM(L, kClsDelta, k_delta_paper<true>, grid, bs, rb_smem,
+                      d.scenes, d.block_scene, d.xb, d.cell_start, d.rho,
+                      d.xold, d.nl, d.ncnt, sh.slots, d.xa, d.v);
+    } else {
+      PBF_LAUNCH_SMEM(L, kClsDelta, k_delta_paper<false>, grid, bs, rb_smem,
+                      d.scenes, d.block_scene, d.xb, d.cell_start, d.rho,
+                      d.xold, d.nl, d.ncnt, sh.slots, d.xa, d.v);
+    }
+  }
+  if (sh.vorticity) {
     PBF_LAUNCH(L, kClsXsph, k_xsph_omega, grid, bs, d.scenes, d.block_scene,
-               cur, d.cell_start, d.v, d.vx, d.om);
-    PBF_LAUNCH(L, kClsVort, k_vort_final, grid, bs, d.scenes, d.block_scene,
-               cur, d.cell_start, d.vx, d.om, d.pid_s, d.pos, d.vel, d.pid,
-               d.err);
+               d.xa, d.cell_start, d.v, d.vx, d.om, d.xb, d.nl, d.ncnt,
+               sh.slots);
+    PBF_LAUNCH_SMEM(L, kClsVort, k_vort_final, grid, bs, rb_smem, d.scenes,
+                    d.block_scene, d.xb, d.cell_start, d.vx, d.om, d.nl,
+                    d.ncnt, sh.slots, d.pid_s, d.pos, d.vel, d.pid, d.err);
   } else {
-    PBF_LAUNCH(L, kClsXsph, k_xsph_final<PH>, grid, bs, d.scenes,
-               d.block_scene, cur, d.cell_start, d.v, d.pid_s, d.pos, d.vel,
+    PBF_LAUNCH(L, kClsXsph, k_xsph_final_paper, grid, bs, d.scenes,
+               d.block_scene, d.xa, d.cell_start, d.v, d.pid_s, d.pos, d.vel,
                d.pid, d.err);
   }
 }
@@ -918,9 +1163,9 @@ void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   KernelTimer *timer) {
   Launcher L{sh, st, timer};
   if (sh.physics == kRef)
-    step_impl<kRef>(d, L);
+    step_ref(d, L);
   else
-    step_impl<kPaper>(d, L);
+    step_paper(d, L);
 }
 
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
@@ -930,14 +1175,17 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
     sort_stage<kRef>(d, L, 1);
     k_count_nbrs<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                             d.cell_start, d.nbr);
-    k_lambda<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
-                                        d.cell_start, d.lam, d.rho);
+    k_lambda_ref<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                      d.cell_start, d.lam, d.rho);
   } else {
     sort_stage<kPaper>(d, L, 1);
     k_count_nbrs<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                               d.cell_start, d.nbr);
-    k_lambda<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
-                                          d.cell_start, d.lam, d.rho);
+    k_lambda_paper<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                        d.cell_start, d.xb, d.rho, d.nl,
+                                        d.ncnt, sh.slots);
+    k_unpack_lambda<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xb,
+                                         d.lam);
   }
 }
 

@@ -12,6 +12,10 @@ namespace pbf {
 // scene of a work-group is one scalar load.
 constexpr int kSlotAlign = 256;
 
+// Capacity of a particle's hit list (neighbours within h).
+constexpr int kMaxHits = 64;
+static_assert(kMaxHits % 8 == 0, "hit lists are stored in groups of 8");
+
 // Per-scene constants, resident in device memory and read by the kernels
 // with scalar loads (the scene index is uniform per work-group).  REF
 // scenes hold the 6-significant-digit constants of src/main.cpp:149-168.
@@ -64,6 +68,8 @@ struct DevBuffers {
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
   ushort4 *xh;    // mixed precision: FP16 cell-relative x* (sorted)
+  uint4 *nl;      // hit lists: kMaxHits/8 groups of 8 x 16-bit entries per slot
+  int *ncnt;      // hits per slot (> kMaxHits: list overflowed)
   int *nbr;       // probe: neighbour counts (sorted)
   double *red;    // diagnostics partials
   int *err;       // device error word
