@@ -530,6 +530,13 @@ int orc_set_threads(int nthreads) {
 #endif
 }
 
+// Number of cells of the grid the oracle bins on; cell_start of
+// orc_probe_f32 has this many entries + 1.
+long orc_num_cells(const pbf_params *p) {
+  if (!params_ok(p, 0)) return -1;
+  return make_consts<double>(*p).C;
+}
+
 // Constants the kernels see: [h, h2, poly6, spiky, dt, rho0, cell_len x3,
 // min x3, max x3, ncell x3, W(dq)] (main.cpp:149-168 rounding in REF).
 int orc_constants(const pbf_params *p, float *out) {

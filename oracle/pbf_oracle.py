@@ -30,6 +30,8 @@ def load():
         build()
     lib = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
+    lib.orc_num_cells.argtypes = [vp]
+    lib.orc_num_cells.restype = ctypes.c_long
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_set_threads.restype = ctypes.c_int
     lib.orc_constants.argtypes = [vp, vp]
@@ -95,8 +97,19 @@ def run_f64(params, pos4, vel4, nsteps: int, wave: float = 0.0):
     return pos, vel
 
 
-def probe_f32(params, pos4, vel4, ncells: int) -> dict:
+def num_cells(params) -> int:
+    c = load().orc_num_cells(_pp(params))
+    assert c > 0, "oracle rejected the parameters"
+    return int(c)
+
+
+def probe_f32(params, pos4, vel4, ncells=None) -> dict:
+    """ncells, if given, must equal the oracle's cell count (checked)."""
     n = pos4.shape[0]
+    c = num_cells(params)
+    if ncells is not None and ncells != c:
+        raise ValueError(f"cell count {ncells} != oracle grid {c}")
+    ncells = c
     pos = np.ascontiguousarray(pos4, np.float32)
     vel = np.ascontiguousarray(vel4, np.float32)
     out = {

@@ -301,20 +301,42 @@ __global__ __launch_bounds__(256) void k_predict_count(
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x = pos[i];
-  float4 v = vel[i];
-  float3 p = make_float3(x.x, x.y, x.z);
-  if (predict) p = predict_exact(sc, x, v);
-  int cx, cy, cz;
-  cell_coords<PH>(sc, p.x, p.y, p.z, cx, cy, cz);
-  cx = min(max(cx, 0), sc.nc[0] - 1);
-  cy = min(max(cy, 0), sc.nc[1] - 1);
-  cz = min(max(cz, 0), sc.nc[2] - 1);
-  const int k = sc.cell_base + cx + (cy + cz * sc.nc[1]) * sc.nc[0];
-  key[i] = k;
-  rank[i] = atomicAdd(&count[k], 1);
-  pred_u[i] = make_float4(p.x, p.y, p.z, v.w);
+  // Padding slots sit at the end of a scene's range and a wave never spans
+  // two scenes, so the valid lanes of a wave are a prefix of it.  They stay
+  // in the wave (no early return) for the cross-lane steps below.
+  const bool valid = i - sc.slot_off < sc.n;
+  int k = -1;
+  if (valid) {
+    const float4 x = pos[i];
+    float4 v = vel[i];
+    float3 p = make_float3(x.x, x.y, x.z);
+    if (predict) p = predict_exact(sc, x, v);
+    int cx, cy, cz;
+    cell_coords<PH>(sc, p.x, p.y, p.z, cx, cy, cz);
+    cx = min(max(cx, 0), sc.nc[0] - 1);
+    cy = min(max(cy, 0), sc.nc[1] - 1);
+    cz = min(max(cz, 0), sc.nc[2] - 1);
+    k = sc.cell_base + cx + (cy + cz * sc.nc[1]) * sc.nc[0];
+    key[i] = k;
+    pred_u[i] = make_float4(p.x, p.y, p.z, v.w);
+  }
+  // Slots arrive in the previous step's cell order, so neighbouring lanes
+  // mostly share a cell: one atomic per run of equal keys, the run's first
+  // lane adding the run length and handing out ranks.
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(k, 1, 64);
+  const bool head = valid && (lane == 0 || prev != k);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long stops = heads | ~__ballot(valid);
+  const unsigned long long upto =
+      lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const int first = 63 - __clzll(heads & upto);  // valid lanes only
+  const unsigned long long after = stops & ~upto;
+  const int next = after ? __ffsll((long long)after) - 1 : 64;
+  int base = 0;
+  if (head) base = atomicAdd(&count[k], next - lane);
+  base = __shfl(base, valid ? first : 0, 64);
+  if (valid) rank[i] = base + lane - first;
 }
 
 // Exclusive scan over a 256-thread block; *total receives the block sum.

@@ -73,6 +73,25 @@ def test_probe_c0_ref_bit_exact(c0_ref):
     assert np.array_equal(g["lambda"], o["lambda"])
 
 
+def test_probe_paper_counts_equal_brute_force():
+    """At the binned positions the 27-cell search finds every pair within h:
+    the PAPER neighbour counts are the exact all-pairs counts."""
+    sc = scenarios.config(0, physics=pbf.PHYSICS_PAPER)
+    say("probe c0 paper", sc.n)
+    p, parts = sc.params[0], sc.particles[0]
+    with pbf.Simulation(p, parts) as s:
+        g = s.probe()
+        assert s.num_cells() == 40 * 20 * 20
+    brute = orc.brute_neighbors(g["pred"], float(np.float32(0.1) ** 2))
+    assert np.array_equal(g["nbr_count"], brute)
+    assert brute.mean() > 20
+    pos, vel = orc.state_from_particles(parts)
+    o = orc.probe_f32(p, pos, vel)
+    assert np.allclose(g["rho"], o["rho"], rtol=2e-5)
+    assert np.allclose(g["lambda"], o["lambda"], rtol=2e-4,
+                       atol=2e-4 * np.abs(o["lambda"]).max())
+
+
 def test_probe_after_steps_bit_exact(c0_ref):
     """After 10 GPU steps (sorted, moved state) the search still agrees with
     the oracle applied to the GPU's own state."""

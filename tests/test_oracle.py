@@ -102,7 +102,7 @@ def test_lattice_density_and_lambda_closed_form(physics):
     h = 0.125
     p = params(box=(2.0, 2.0, 2.0), n=(16, 16, 16), physics=physics, gravity_y=0.0)
     pos, vel, ci = _lattice_state(h)
-    out = orc.probe_f32(p, pos, vel, 16 ** 3)
+    out = orc.probe_f32(p, pos, vel)
     rho0 = 6378.0
     if physics == pbf.PHYSICS_REF:
         poly6 = float(fmt6(315.0 / (64.0 * math.pi * h ** 9)))
@@ -139,7 +139,7 @@ def test_survey_lattice_numbers_h01():
     p = params(box=(4.0, 4.0, 4.0), n=(40, 40, 40), physics=pbf.PHYSICS_PAPER,
                gravity_y=0.0)
     pos, vel, ci = _lattice_state(h=0.1, n=9, origin=1.0)
-    out = orc.probe_f32(p, pos, vel, 40 ** 3)
+    out = orc.probe_f32(p, pos, vel)
     if out["nbr_count"][ci] == 26:
         assert out["rho"][ci] == pytest.approx(6511.5199, rel=1e-5)
         assert out["lambda"][ci] == pytest.approx(-2.507166e-5, rel=1e-4)
