@@ -134,6 +134,42 @@ def rocprof_summary(root: str, particles: int, copy_mib: int) -> str:
     return "\n".join(out)
 
 
+def batch_vs_sequential(physics: int, steps: int, nscenes: int = 256):
+    """256 independent scenes (N_s log-uniform in [2048, 65536]) advanced by
+    one batched handle (one launch per kernel for all scenes) against the
+    same scenes as separate handles stepped one after another."""
+    sc = scenarios.batch_config(nscenes=nscenes, physics=physics)
+    n = sc.n
+    batch = pbf.BatchSimulation(sc.params, sc.particles)
+    batch.step(2)
+    batch.synchronize()
+    ms_batch = batch.time_steps(steps)
+    batch.close()
+    sims = [pbf.Simulation(p, parts) for p, parts in zip(sc.params, sc.particles)]
+    for s in sims:
+        s.step(1)
+    for s in sims:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.step(steps)
+    for s in sims:
+        s.synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    for s in sims:
+        s.close()
+    return {
+        "scenes": nscenes,
+        "particles": n,
+        "steps": steps,
+        "batched_particle_steps_per_s": n * steps / (ms_batch / 1e3),
+        "sequential_particle_steps_per_s": n * steps / (ms_seq / 1e3),
+        "speedup": ms_seq / ms_batch,
+        "batched_ms_per_step": ms_batch / steps,
+        "sequential_ms_per_step": ms_seq / steps,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +181,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="steps of the 256-scene batch vs sequential "
+                         "comparison (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain float4 copy of MIB MiB (counter "
                          "calibration) and print its time")
@@ -252,6 +291,10 @@ def main():
             "mean_abs_density_err": diag["mean_abs_density_err"],
         },
     }
+    if args.batch_steps > 0:
+        print(f"[bench] 256-scene batch vs sequential, {args.batch_steps} steps",
+              file=sys.stderr, flush=True)
+        out["batch"] = batch_vs_sequential(physics, args.batch_steps)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cps, secs = cpu_baseline(params, parts, args.cpu_steps, threads)

@@ -78,7 +78,7 @@ typedef struct pbf_params {
   int32_t precision;      /* PBF_PRECISION_*                                */
   int32_t fp32_iterations;/* mixed precision: trailing full-FP32 iterations */
   float sphere[4];        /* obstacle sphere (cx, cy, cz, r); r <= 0: none  */
-  int32_t block_size;     /* gather work-group size; 0 = autotuned          */
+  int32_t block_size;     /* gather work-group size 64/128/256; 0 = tuned   */
   int32_t reserved[7];
 } pbf_params;
 
@@ -214,9 +214,12 @@ int pbf_time_steps(pbf_sim *sim, uint32_t nsteps, double *ms);
  * the per-kernel FETCH_SIZE / WRITE_SIZE readings. */
 int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms);
 
-/* Launch configuration chosen for this handle (after autotuning):
- * cfg[0] = gather block size, cfg[1] = particles per gather work-item
- * (1), cfg[2] = sort digit width in bits (0 = one-pass counting sort). */
+/* Launch configuration of this handle: cfg[0] = gather work-group size
+ * (chosen by timing 64, 128 and 256 on the first step when block_size is
+ * 0 and the handle has at least 2^18 slots; cached per shape, and in the
+ * file named by the PBF_TUNE_CACHE environment variable), cfg[1] =
+ * particles per work-item (1), cfg[2] = sort passes (1: one counting pass),
+ * cfg[3] = FP16 iterations per step. */
 int pbf_get_config(const pbf_sim *sim, int32_t cfg[4]);
 
 #ifdef __cplusplus

@@ -12,6 +12,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -212,6 +216,7 @@ struct pbf_sim {
   hipGraphExec_t graph = nullptr;
   double *diag_out = nullptr;
   float wave = 0.0f;
+  bool tune_pending = false;  // block size still to be chosen
 
   template <typename T>
   T *alloc(size_t count) {
@@ -309,7 +314,8 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   for (uint32_t s = 0; s < nscenes; ++s)
     vort = vort || params[s].vorticity_eps != 0.0f;
   sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
-  sh.block = p0.block_size ? p0.block_size : 256;
+  sh.block = p0.block_size ? p0.block_size : 128;
+  S.tune_pending = p0.block_size == 0;
 
   const size_t P = (size_t)sh.slots, M = (size_t)sh.cells;
   pbf::DevBuffers &d = S.d;
@@ -374,7 +380,125 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   PBF_HIP(hipStreamSynchronize(S.stream));
 }
 
+// ---- autotuner ---------------------------------------------------------
+// Chooses the work-group size of the gather kernels (64, 128 or 256) by
+// timing two steps of each on the handle's own state, saved before and
+// restored after.  Results are kept per problem shape for the process and,
+// when PBF_TUNE_CACHE names a file, in that file ("key block" lines).
+// Handles below kTuneMinSlots slots keep the default: there the launch
+// overhead, not the kernels, dominates.
+constexpr int kTuneMinSlots = 1 << 18;
+
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune;
+
+std::string tune_key(const pbf_sim &S) {
+  hipDeviceProp_t prop;
+  std::string arch = "gpu";
+  if (hipGetDeviceProperties(&prop, S.device) == hipSuccess)
+    arch = prop.gcnArchName;
+  std::ostringstream k;
+  k << arch << ":" << S.shape.slots << ":" << S.shape.cells << ":"
+    << S.shape.physics << ":" << S.shape.iterations << ":"
+    << S.shape.vorticity << ":" << S.scenes.size();
+  return k.str();
+}
+
+int tune_lookup(const std::string &key) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second;
+  if (const char *path = std::getenv("PBF_TUNE_CACHE")) {
+    std::ifstream f(path);
+    std::string k;
+    int b;
+    while (f >> k >> b)
+      if (k == key && (b == 64 || b == 128 || b == 256)) {
+        g_tune[key] = b;
+        return b;
+      }
+  }
+  return 0;
+}
+
+void tune_store(const std::string &key, int block) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune[key] = block;
+  if (const char *path = std::getenv("PBF_TUNE_CACHE")) {
+    std::ofstream f(path, std::ios::app);
+    f << key << " " << block << "\n";
+  }
+}
+
+void autotune(pbf_sim &S) {
+  if (!S.tune_pending) return;
+  S.tune_pending = false;
+  if (S.shape.slots < kTuneMinSlots) return;
+  const std::string key = tune_key(S);
+  if (const int b = tune_lookup(key)) {
+    S.shape.block = b;
+    return;
+  }
+  const size_t P = (size_t)S.shape.slots;
+  float4 *spos = nullptr, *svel = nullptr;
+  int *spid = nullptr;
+  auto release = [&] {
+    if (spos) (void)hipFree(spos);
+    if (svel) (void)hipFree(svel);
+    if (spid) (void)hipFree(spid);
+  };
+  auto copy = [&](void *dst, const void *src, size_t bytes) {
+    PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S.stream));
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  try {
+    PBF_HIP(hipMalloc(&spos, P * sizeof(float4)));
+    PBF_HIP(hipMalloc(&svel, P * sizeof(float4)));
+    PBF_HIP(hipMalloc(&spid, P * sizeof(int)));
+    PBF_HIP(hipEventCreate(&e0));
+    PBF_HIP(hipEventCreate(&e1));
+    copy(spos, S.d.pos, P * sizeof(float4));
+    copy(svel, S.d.vel, P * sizeof(float4));
+    copy(spid, S.d.pid, P * sizeof(int));
+    float best = 1e30f;
+    int best_b = S.shape.block;
+    for (int b : {64, 128, 256}) {
+      S.shape.block = b;
+      copy(S.d.pos, spos, P * sizeof(float4));
+      copy(S.d.vel, svel, P * sizeof(float4));
+      copy(S.d.pid, spid, P * sizeof(int));
+      pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);  // warm-up
+      PBF_HIP(hipEventRecord(e0, S.stream));
+      for (int t = 0; t < 2; ++t)
+        pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);
+      PBF_HIP(hipEventRecord(e1, S.stream));
+      PBF_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      PBF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_b = b;
+      }
+    }
+    copy(S.d.pos, spos, P * sizeof(float4));
+    copy(S.d.vel, svel, P * sizeof(float4));
+    copy(S.d.pid, spid, P * sizeof(int));
+    PBF_HIP(hipStreamSynchronize(S.stream));
+    S.shape.block = best_b;
+    tune_store(key, best_b);
+  } catch (...) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    release();
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  release();
+}
+
 void ensure_graph(pbf_sim &S) {
+  autotune(S);
   if (S.graph) return;
   hipGraph_t g = nullptr;
   PBF_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
@@ -667,6 +791,7 @@ int pbf_profile_steps(pbf_sim *S, uint32_t nsteps, double *ms,
     if (S->shape.slots == 0) return;
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
+    autotune(*S);
     EventTimer timer;
     for (uint32_t t = 0; t < nsteps; ++t)
       pbf::enqueue_step(S->d, S->shape, S->stream, &timer);
@@ -748,7 +873,7 @@ int pbf_get_config(const pbf_sim *S, int32_t cfg[4]) {
     need(S && cfg, "NULL argument");
     cfg[0] = S->shape.block;
     cfg[1] = 1;
-    cfg[2] = 0;
+    cfg[2] = 1;
     cfg[3] = S->shape.fp16_iterations;
   });
 }
