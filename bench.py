@@ -170,6 +170,86 @@ def batch_vs_sequential(physics: int, steps: int, nscenes: int = 256):
     }
 
 
+def _run_to(params, parts, steps: int, timed: int, warmup: int):
+    """Advance a fresh handle `steps` steps, timing `timed` of them (graph
+    launches between HIP events) after `warmup`; returns (ms of the timed
+    steps, diagnostics, positions) at step `steps`."""
+    sim = pbf.Simulation(params, parts)
+    sim.step(warmup)
+    sim.synchronize()
+    ms = sim.time_steps(timed)
+    rest = steps - warmup - timed
+    if rest > 0:
+        sim.step(rest)
+    diag = sim.diagnostics()
+    pos, _ = sim.dump()
+    sim.close()
+    return ms, diag, pos
+
+
+def mixed_vs_plain(timed: int, oracle_threads: int, with_oracle: bool):
+    """The 4,194,304-particle sphere dam (c4), PAPER physics: FP32 against
+    mixed precision (3 FP16-gather iterations + 1 FP32 iteration), both run
+    50 steps from the same input; the speed of each over `timed` graph
+    launches, the GPU FP64 density error after 50 steps, and the position
+    RMS between the two.  The FP64 oracle comparison runs on a 65,536-
+    particle replica of the same geometry (64 x 32 x 32 block, 6 x 2 x 2
+    box, seeded sphere) for 50 steps."""
+    sc = scenarios.config(4)
+    p32 = sc.params[0]
+    pmx = p32.copy()
+    pmx.precision, pmx.fp32_iterations = pbf.PRECISION_MIXED, 1
+    parts = sc.particles[0]
+    warm = 5
+    ms32, d32, x32 = _run_to(p32, parts, sc.steps, timed, warm)
+    msmx, dmx, xmx = _run_to(pmx, parts, sc.steps, timed, warm)
+    n = sc.n
+    out = {
+        "workload": (f"c4: sphere dam, {n} particles, {p32.iterations} "
+                     f"iterations, paper physics, sphere r = "
+                     f"{sc.meta['sphere'][3]:.3f}"),
+        "particles": n,
+        "timed_steps": timed,
+        "fp32_particle_steps_per_s": n * timed / (ms32 / 1e3),
+        "mixed_particle_steps_per_s": n * timed / (msmx / 1e3),
+        "speedup": ms32 / msmx,
+        "fp32_max_density_err": d32["max_compression"],
+        "mixed_max_density_err": dmx["max_compression"],
+        "fp32_mean_density_err": d32["mean_abs_density_err"],
+        "mixed_mean_density_err": dmx["mean_abs_density_err"],
+        "position_rms_mixed_vs_fp32": float(np.sqrt(np.mean(
+            np.sum((xmx[:, :3].astype(np.float64) - x32[:, :3]) ** 2, axis=1)))),
+        "steps": sc.steps,
+    }
+    if with_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pbf_oracle as orc
+
+        orc.set_threads(oracle_threads)
+        small = scenarios.sphere_dam((64, 32, 32), (6.0, 2.0, 2.0), seed=1005,
+                                     steps=50)
+        q32 = small.params[0]
+        qmx = q32.copy()
+        qmx.precision, qmx.fp32_iterations = pbf.PRECISION_MIXED, 1
+        sp = small.particles[0]
+        pos, vel = orc.state_from_particles(sp, np.float64)
+        t0 = time.perf_counter()
+        op, _ = orc.run_f64(q32, pos, vel, small.steps)
+        secs = time.perf_counter() - t0
+        rho = orc.density_f64(q32, op)
+        res = {"particles": small.n, "steps": small.steps,
+               "oracle_seconds": secs, "oracle_threads": oracle_threads,
+               "fp64_max_density_err": float(np.abs(rho / q32.rest_density - 1).max())}
+        for tag, q in (("fp32", q32), ("mixed", qmx)):
+            _, d, x = _run_to(q, sp, small.steps, 1, 1)
+            e = x[:, :3].astype(np.float64) - op[:, :3]
+            res[f"{tag}_position_rms_vs_fp64"] = float(np.sqrt(np.mean(np.sum(e * e, axis=1))))
+            res[f"{tag}_position_max_vs_fp64"] = float(np.abs(e).max())
+            res[f"{tag}_max_density_err"] = d["max_compression"]
+        out["vs_fp64_oracle"] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +263,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="steps of the 256-scene batch vs sequential "
+                         "comparison (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=20,
+                    help="timed steps of the c4 mixed-precision vs FP32 "
                          "comparison (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain float4 copy of MIB MiB (counter "
@@ -295,6 +378,11 @@ def main():
         print(f"[bench] 256-scene batch vs sequential, {args.batch_steps} steps",
               file=sys.stderr, flush=True)
         out["batch"] = batch_vs_sequential(physics, args.batch_steps)
+    if args.mixed_steps > 0 and physics == pbf.PHYSICS_PAPER:
+        print(f"[bench] c4 mixed precision vs FP32, {args.mixed_steps} timed steps",
+              file=sys.stderr, flush=True)
+        out["mixed"] = mixed_vs_plain(args.mixed_steps, min(16, os.cpu_count() or 1),
+                                      not args.no_cpu)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cps, secs = cpu_baseline(params, parts, args.cpu_steps, threads)

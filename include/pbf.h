@@ -184,7 +184,10 @@ int pbf_probe(pbf_sim *sim, int32_t *cell_of, int32_t *sorted_ids,
               int32_t *cell_start, int32_t *nbr_count, float *rho,
               float *lambda, float *pred4);
 
-/* Number of cells C of a single-scene handle (length of cell_start is C+1). */
+/* Number of cells C of a single-scene handle (length of cell_start is C+1).
+ * REF handles bin in the x_n * y_n * z_n cells of the parameters; PAPER
+ * handles bin in 4 x_n * y_n * z_n cells (each x cell split in 4 along x, so
+ * that the neighbour search can cut its rows to the reach h). */
 uint32_t pbf_num_cells(const pbf_sim *sim);
 
 /* FP64 reductions on the device over the current state (all scenes):

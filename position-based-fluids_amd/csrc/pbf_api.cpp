@@ -94,8 +94,15 @@ void validate(const pbf_params &p) {
     invalid("mixed precision needs 1 <= fp32_iterations <= iterations");
   if (p.precision == PBF_PRECISION_MIXED && p.physics != PBF_PHYSICS_PAPER)
     invalid("mixed precision is available for PAPER physics only");
-  if (p.precision == PBF_PRECISION_MIXED)
-    invalid("mixed precision is not available in this build");
+  if (p.physics == PBF_PHYSICS_PAPER) {
+    // the 3 x 3 rows of the search need y and z cells at least h = x cell
+    const double hx = ((double)p.x_max - p.x_min) / p.x_n;
+    const double cy = ((double)p.y_max - p.y_min) / p.y_n;
+    const double cz = ((double)p.z_max - p.z_min) / p.z_n;
+    if (cy < hx * (1.0 - 1e-6) || cz < hx * (1.0 - 1e-6))
+      invalid("PAPER physics needs y and z cells at least as long as the "
+              "x cells (h)");
+  }
   if (!(p.scorr_n >= 0 && p.scorr_n <= 16) || !is_int_valued(p.scorr_n))
     invalid("scorr_n must be an integer in [0, 16]");
   if (p.block_size != 0 && p.block_size != 64 && p.block_size != 128 &&
@@ -121,8 +128,13 @@ pbf::Scene make_scene(const pbf_params &p) {
       s.bmax[a] = bmax[a];
       s.cell_len[a] = (float)(((double)bmax[a] - (double)bmin[a]) / n[a]);
     }
-    s.inv_cell[a] = 1.0f / s.cell_len[a];
     s.nc[a] = (int)n[a];
+    if (!ref && a == 0) {  // fine x bins of the PAPER search
+      s.nc[0] *= pbf::kXSub;
+      s.cell_len[0] = (float)(((double)bmax[0] - (double)bmin[0]) /
+                              ((double)n[0] * pbf::kXSub));
+    }
+    s.inv_cell[a] = 1.0f / s.cell_len[a];
   }
   s.ncells = s.nc[0] * s.nc[1] * s.nc[2];
   const double pi = 3.14159265358979323846;
@@ -143,6 +155,8 @@ pbf::Scene make_scene(const pbf_params &p) {
     s.spiky = (float)(45.0 / (pi * std::pow(h, 6)));
     s.dt = p.time_step_length;
     s.rho0 = p.rest_density;
+    const double reach = h * (1.0 + (double)pbf::kReachMargin);
+    s.reach2 = (float)(reach * reach);
   }
   s.inv_rho0 = 1.0f / s.rho0;
   s.eps = p.relaxation;
@@ -158,7 +172,6 @@ pbf::Scene make_scene(const pbf_params &p) {
   s.vort_eps = p.vorticity_eps;
   s.wave = 0.0f;
   for (int a = 0; a < 4; ++a) s.sphere[a] = p.sphere[a];
-  s.half_scale = (float)(1.0 / h);
   return s;
 }
 
@@ -310,6 +323,11 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   sh.fp16_iterations = p0.precision == PBF_PRECISION_MIXED
                            ? p0.iterations - p0.fp32_iterations
                            : 0;
+  if (sh.fp16_iterations > 0)
+    for (const pbf::Scene &sc : S.scenes)
+      if (sc.nc[0] > 65535)  // the FP16 records hold the fine x index in 16 bits
+        invalid("mixed precision supports at most 65535 fine x cells (" +
+                std::to_string(sc.nc[0] / pbf::kXSub) + " x cells given)");
   bool vort = false;
   for (uint32_t s = 0; s < nscenes; ++s)
     vort = vort || params[s].vorticity_eps != 0.0f;
@@ -340,6 +358,10 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   if (sh.vorticity) {
     d.vx = S.alloc<float4>(P);
     d.om = S.alloc<float4>(P);
+  }
+  if (sh.fp16_iterations > 0) {
+    d.xh = S.alloc<uint2>(P);
+    d.kc = S.alloc<int>(P);
   }
   d.nl = S.alloc<uint4>(P * (size_t)(pbf::kMaxHits / 8));
   d.ncnt = S.alloc<int>(P);

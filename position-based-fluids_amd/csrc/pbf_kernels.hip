@@ -22,6 +22,8 @@
 //                  xsph_omega + vort_final.
 #include "pbf_kernels.hpp"
 
+#include <hip/hip_fp16.h>
+
 #include <algorithm>
 
 namespace pbf {
@@ -54,6 +56,29 @@ __device__ __forceinline__ int trunc_int(float v) {
   return (int)v;
 }
 
+// Mixed precision: a position as FP16 offsets from the origin of its bin
+// (fine x cell, y/z cell), packed (x | y << 16, z | fx << 16) in 8 bytes
+// with fx the bin's fine x index (< 65536: checked at set-up).
+__device__ __forceinline__ uint2 pack_rel(const Scene &sc, float3 x, int c) {
+  const int local = c - sc.cell_base;
+  const int fx = local % sc.nc[0], t = local / sc.nc[0];
+  const int cy = t % sc.nc[1], cz = t / sc.nc[1];
+  const float ox = sc.bmin[0] + fx * sc.cell_len[0];
+  const float oy = sc.bmin[1] + cy * sc.cell_len[1];
+  const float oz = sc.bmin[2] + cz * sc.cell_len[2];
+  const unsigned hx = __half_as_ushort(__float2half_rn(x.x - ox));
+  const unsigned hy = __half_as_ushort(__float2half_rn(x.y - oy));
+  const unsigned hz = __half_as_ushort(__float2half_rn(x.z - oz));
+  return make_uint2(hx | (hy << 16), hz | ((unsigned)fx << 16));
+}
+
+__device__ __forceinline__ float half_lo(unsigned w) {
+  return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu)));
+}
+__device__ __forceinline__ float half_hi(unsigned w) {
+  return __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+}
+
 // Cell coordinates of a position.  REF: (int)((x - SYSTEM_MIN) /
 // CELL_LENGTH) with IEEE division (update_cells.cl:9-15).  PAPER: the same
 // with a multiply by the reciprocal.
@@ -72,16 +97,65 @@ __device__ __forceinline__ void cell_coords(const Scene &sc, float x, float y,
   }
 }
 
+// PAPER search rows.  The stencil is the 3 x 3 (y, z) cells around the
+// particle's CURRENT cell (cells from its current position, lists from the
+// step's sort, cells outside the grid skipped: compute_scaling.cl:21-53);
+// each (y, z) cell is a row of fine x cells, contiguous in sorted order.  A
+// row is cut to the fine x cells within the reach R = (1 + margin) h:
+// with d the distance from xi to the row's y-z rectangle, the fine cells
+// that meet [x - s, x + s], s = sqrt(R^2 - d^2), and no cells if d >= R.
+// Every particle whose bin lies within R of xi is visited.  Returns the
+// sorted slot range [b, e) (b == e: nothing to visit).  A replaying pass
+// recomputes the row starts of the searching pass from the same position, so
+// the arithmetic is pinned: no contraction into FMAs, which the compiler
+// would otherwise choose per call site.
+__device__ __forceinline__ void row_range(const Scene &sc,
+                                          const int *__restrict__ cs,
+                                          float3 xi, int yy, int zz, int &b,
+                                          int &e) {
+#pragma clang fp contract(off)
+  const float y0 = sc.bmin[1] + (float)yy * sc.cell_len[1];
+  const float z0 = sc.bmin[2] + (float)zz * sc.cell_len[2];
+  const float dy = fmaxf(fmaxf(y0 - xi.y, xi.y - (y0 + sc.cell_len[1])), 0.0f);
+  const float dz = fmaxf(fmaxf(z0 - xi.z, xi.z - (z0 + sc.cell_len[2])), 0.0f);
+  const float rem = sc.reach2 - dy * dy - dz * dz;
+  b = e = 0;
+  if (rem > 0.0f) {
+    const float u = (xi.x - sc.bmin[0]) * sc.inv_cell[0];
+    const float s = __builtin_sqrtf(rem) * sc.inv_cell[0];
+    // clamp in float first: xi may lie far outside the grid
+    const float lo = fmaxf(floorf(u - s), 0.0f);
+    const float hi = fminf(floorf(u + s), (float)(sc.nc[0] - 1));
+    if (lo <= hi) {
+      const int base = sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0];
+      b = cs[base + (int)lo] + sc.shift;
+      e = cs[base + (int)hi + 1] + sc.shift;
+    }
+  }
+}
+
 // Visit every slot of the 27-cell stencil of a particle at xi (cells from
 // its CURRENT position, lists from the step's sort, neighbour cells outside
-// the grid skipped: compute_scaling.cl:21-53).  Rows (dz, dy) each give one
-// contiguous range over the x-cells [cx-1, cx+1] clipped to the grid.
+// the grid skipped: compute_scaling.cl:21-53).  REF: rows (dz, dy) each give
+// one contiguous range over the x-cells [cx-1, cx+1] clipped to the grid.
+// PAPER: the rows cut to the reach (row_range).
 template <int PH, typename F>
 __device__ __forceinline__ void for_stencil(const Scene &sc,
                                             const int *__restrict__ cs,
                                             float x, float y, float z, F &&f) {
   int cx, cy, cz;
   cell_coords<PH>(sc, x, y, z, cx, cy, cz);
+  if constexpr (PH == kPaper) {
+    const float3 xi = make_float3(x, y, z);
+    for (int zz = max(cz - 1, 0); zz <= min(cz + 1, sc.nc[2] - 1); ++zz) {
+      for (int yy = max(cy - 1, 0); yy <= min(cy + 1, sc.nc[1] - 1); ++yy) {
+        int b, e;
+        row_range(sc, cs, xi, yy, zz, b, e);
+        for (int j = b; j < e; ++j) f(j);
+      }
+    }
+    return;
+  }
   const int xlo = max(cx - 1, 0), xhi = min(cx + 1, sc.nc[0] - 1);
   if (xlo > xhi) return;
   const int ylo = max(cy - 1, 0), yhi = min(cy + 1, sc.nc[1] - 1);
@@ -455,7 +529,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     const int *__restrict__ key, const int *__restrict__ cell_start,
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
     int *__restrict__ pid_s, float4 *__restrict__ xa,
-    float4 *__restrict__ xold) {
+    float4 *__restrict__ xold, int *__restrict__ kc, uint2 *__restrict__ xh) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int t = lb * blockDim.x + threadIdx.x;
@@ -472,6 +546,10 @@ __global__ __launch_bounds__(256) void k_reorder(
   pid_s[f] = p;
   xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
   xold[f] = make_float4(x.x, x.y, x.z, pu.w);
+  if (xh) {  // mixed precision: sorted cell id and FP16 cell-relative x*
+    kc[f] = c;
+    xh[f] = pack_rel(sc, make_float3(pu.x, pu.y, pu.z), c);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -541,17 +619,14 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
                                           float3 xi, F &&f) {
   int cx, cy, cz;
   cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
-  const int xlo = max(cx - 1, 0), xhi = min(cx + 1, sc.nc[0] - 1);
-  if (xlo > xhi) return;
   for (int dz = 0; dz < 3; ++dz) {
     const int zz = cz - 1 + dz;
     if (zz < 0 || zz >= sc.nc[2]) continue;
     for (int dy = 0; dy < 3; ++dy) {
       const int yy = cy - 1 + dy;
       if (yy < 0 || yy >= sc.nc[1]) continue;
-      const int base = sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0];
-      const int b = cs[base + xlo] + sc.shift;
-      const int e = cs[base + xhi + 1] + sc.shift;
+      int b, e;
+      row_range(sc, cs, xi, yy, zz, b, e);
       const int row = dz * 3 + dy;
       int j = b;
       for (; j + 4 <= e; j += 4) {
@@ -583,14 +658,12 @@ __device__ __forceinline__ void replay_hits(const Scene &sc,
     const int lane = threadIdx.x, bs = blockDim.x;
     int cx, cy, cz;
     cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
-    const int xlo = max(cx - 1, 0);
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
       const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
-      int start = 0;
+      int start = 0, end = 0;
       if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1])
-        start = cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + xlo] +
-                sc.shift;
+        row_range(sc, cs, xi, yy, zz, start, end);
       rb[r * bs + lane] = start;
     }
     for (int g = 0; g * 8 < cnt; ++g) {
@@ -760,15 +833,96 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
   rho[i] = dens;
 }
 
+// Mixed precision: the same pass over FP16 bin-relative positions: the rows
+// and the recorded hits are those of the FP32 pass; a candidate's bin origin
+// is its row's y/z corner and its fine x index times the fine cell length:
+// r = (x_i - origin) - offset_j, evaluated and summed in FP32.
+__global__ __launch_bounds__(256) void k_lambda_paper_h(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ A, const uint2 *__restrict__ XH,
+    const int *__restrict__ cs, float4 *__restrict__ B,
+    float *__restrict__ rho, uint4 *__restrict__ nl, int *__restrict__ ncnt,
+    int stride) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 x4 = A[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
+  float wsum = 0.0f, g2 = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  const float h = sc.h, h2 = sc.h2;
+  // The self pair is skipped by index: its FP16 offset differs from the FP32
+  // one by the rounding, so r^2 > 0 would admit it, and the delta pass
+  // replaying the hit at r = 0 would divide by zero.
+  auto pair = [&](int j, float rx, float ry, float rz, int row, int b) {
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (j != i && r2 > 0.0f && r2 < h2) {
+      rec.add(j, row, b, nl, stride, i);
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float t = h2 - r2;
+      const float hr2 = hr * hr;
+      wsum += t * t * t;
+      g2 += hr2 * hr2;
+      const float gs = hr2 * inv;
+      gx += rx * gs;
+      gy += ry * gs;
+      gz += rz * gs;
+    }
+  };
+  int cx, cy, cz;
+  cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
+  const float ux = xi.x - sc.bmin[0], wx = sc.cell_len[0];
+  auto cand = [&](int j, uint2 w, float qy, float qz, int row, int b) {
+    const float ox = (float)(w.y >> 16) * wx;
+    pair(j, (ux - ox) - half_lo(w.x), qy - half_hi(w.x), qz - half_lo(w.y),
+         row, b);
+  };
+  for (int dz = 0; dz < 3; ++dz) {
+    const int zz = cz - 1 + dz;
+    if (zz < 0 || zz >= sc.nc[2]) continue;
+    const float qz = xi.z - (sc.bmin[2] + zz * sc.cell_len[2]);
+    for (int dy = 0; dy < 3; ++dy) {
+      const int yy = cy - 1 + dy;
+      if (yy < 0 || yy >= sc.nc[1]) continue;
+      const float qy = xi.y - (sc.bmin[1] + yy * sc.cell_len[1]);
+      int b, e;
+      row_range(sc, cs, xi, yy, zz, b, e);
+      const int row = dz * 3 + dy;
+      int j = b;
+      for (; j + 4 <= e; j += 4) {
+        const uint2 w0 = XH[j], w1 = XH[j + 1], w2 = XH[j + 2], w3 = XH[j + 3];
+        cand(j, w0, qy, qz, row, b);
+        cand(j + 1, w1, qy, qz, row, b);
+        cand(j + 2, w2, qy, qz, row, b);
+        cand(j + 3, w3, qy, qz, row, b);
+      }
+      for (; j < e; ++j) cand(j, XH[j], qy, qz, row, b);
+    }
+  }
+  rec.flush(nl, ncnt, stride, i);
+  const float dens = sc.poly6 * wsum;
+  const float sp2 = sc.spiky * sc.spiky;
+  const float denom =
+      sp2 * (gx * gx + gy * gy + gz * gz + g2) * (sc.inv_rho0 * sc.inv_rho0) +
+      sc.eps;
+  B[i] = make_float4(xi.x, xi.y, xi.z, -(dens * sc.inv_rho0 - 1.0f) / denom);
+  rho[i] = dens;
+}
+
 // eqs. 13, 14 and the collision clamp over the hits of the lambda pass.
 // LAST also forms v = (x* - x)/dt and carries rho in .w for XSPH.
-template <bool LAST>
+// WRITE_H (mixed precision) also refreshes the FP16 cell-relative copy for
+// the next FP16 lambda pass.
+template <bool LAST, bool WRITE_H>
 __global__ __launch_bounds__(256) void k_delta_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ B, const int *__restrict__ cs,
     const float *__restrict__ rho, const float4 *__restrict__ xold,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
-    float4 *__restrict__ A, float4 *__restrict__ V) {
+    const int *__restrict__ kc, float4 *__restrict__ A, float4 *__restrict__ V,
+    uint2 *__restrict__ XH) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -780,11 +934,14 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
   const int n = sc.scorr_n;
+  // r^2 > 0 is tested for recorded hits too: an FP16 lambda pass can record
+  // a pair whose FP32 positions coincide (two particles clamped onto the
+  // same wall corner, binned in different cells), whose gradient is zero.
   replay_hits(sc, cs, B, nl, ncnt, stride, i, xi, rb_lds,
               [&](int, float4 xj, bool hit) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (hit || (r2 > 0.0f && r2 < h2)) {
+    if (r2 > 0.0f && (hit || r2 < h2)) {
       const float inv = __builtin_amdgcn_rsqf(r2);
       const float hr = h - r2 * inv;
       const float t = h2 - r2;
@@ -803,6 +960,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   const float3 dp = make_float3(ax * c, ay * c, az * c);
   const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
   A[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
+  if constexpr (WRITE_H) XH[i] = pack_rel(sc, xn, kc[i]);
   if constexpr (LAST) {
     const float4 xo = xold[i];
     const float3 v = velocity_exact(sc, xn, xo);
@@ -1120,7 +1278,7 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
              d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
              d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
-             d.xa, d.xold);
+             d.xa, d.xold, d.kc, sh.fp16_iterations > 0 ? d.xh : nullptr);
 }
 
 static void step_ref(const DevBuffers &d, Launcher &L) {
@@ -1154,17 +1312,30 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
   sort_stage<kPaper>(d, L, 1);
   const size_t rb_smem = (size_t)9 * bs * sizeof(int);  // replay row starts
   for (int it = 0; it < sh.iterations; ++it) {
-    PBF_LAUNCH(L, kClsLambda, k_lambda_paper, grid, bs, d.scenes,
-               d.block_scene, d.xa, d.cell_start, d.xb, d.rho, d.nl, d.ncnt,
-               sh.slots);
-    if (it == sh.iterations - 1) {
-      PBF_LAUNCH_SMEM(L, kClsDelta, k_delta_paper<true>, grid, bs, rb_smem,
-                      d.scenes, d.block_scene, d.xb, d.cell_start, d.rho,
-                      d.xold, d.nl, d.ncnt, sh.slots, d.xa, d.v);
+    if (it < sh.fp16_iterations) {
+      PBF_LAUNCH(L, kClsLambda, k_lambda_paper_h, grid, bs, d.scenes,
+                 d.block_scene, d.xa, d.xh, d.cell_start, d.xb, d.rho, d.nl,
+                 d.ncnt, sh.slots);
     } else {
-      PBF_LAUNCH_SMEM(L, kClsDelta, k_delta_paper<false>, grid, bs, rb_smem,
-                      d.scenes, d.block_scene, d.xb, d.cell_start, d.rho,
-                      d.xold, d.nl, d.ncnt, sh.slots, d.xa, d.v);
+      PBF_LAUNCH(L, kClsLambda, k_lambda_paper, grid, bs, d.scenes,
+                 d.block_scene, d.xa, d.cell_start, d.xb, d.rho, d.nl, d.ncnt,
+                 sh.slots);
+    }
+    if (it == sh.iterations - 1) {
+      PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<true, false>), grid, bs,
+                      rb_smem, d.scenes, d.block_scene, d.xb, d.cell_start,
+                      d.rho, d.xold, d.nl, d.ncnt, sh.slots, d.kc, d.xa, d.v,
+                      d.xh);
+    } else if (it + 1 < sh.fp16_iterations) {
+      PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<false, true>), grid, bs,
+                      rb_smem, d.scenes, d.block_scene, d.xb, d.cell_start,
+                      d.rho, d.xold, d.nl, d.ncnt, sh.slots, d.kc, d.xa, d.v,
+                      d.xh);
+    } else {
+      PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<false, false>), grid, bs,
+                      rb_smem, d.scenes, d.block_scene, d.xb, d.cell_start,
+                      d.rho, d.xold, d.nl, d.ncnt, sh.slots, d.kc, d.xa, d.v,
+                      d.xh);
     }
   }
   if (sh.vorticity) {

@@ -12,6 +12,15 @@ namespace pbf {
 // scene of a work-group is one scalar load.
 constexpr int kSlotAlign = 256;
 
+// PAPER binning: the x cells are kXSub times narrower than the y/z cells
+// (Scene.nc[0] and cell_len[0] describe the fine x cells), so that the
+// candidate range of a stencil row can be cut to the particle's reach.
+constexpr int kXSub = 4;
+
+// PAPER search reach: (1 + kReachMargin) h.  The margin absorbs the motion of
+// particles between the step's sort (which fixes the bins) and a later pass.
+constexpr float kReachMargin = 0.04f;
+
 // Capacity of a particle's hit list (neighbours within h).
 constexpr int kMaxHits = 64;
 static_assert(kMaxHits % 8 == 0, "hit lists are stored in groups of 8");
@@ -37,7 +46,7 @@ struct Scene {
   float vort_eps;
   float wave;          // x-min wall offset (Simulation::setWaveGenerator)
   float sphere[4];     // PAPER obstacle (cx, cy, cz, r), r <= 0: none
-  float half_scale;    // mixed precision: FP16 cell-relative scale
+  float reach2;        // PAPER: ((1 + kReachMargin) h)^2
   int pad[3];
 };
 static_assert(sizeof(Scene) % 16 == 0, "Scene must stay 16-byte aligned");
@@ -67,7 +76,8 @@ struct DevBuffers {
   float4 *v;      // velocity after the constraint solve (sorted), .w = m
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
-  ushort4 *xh;    // mixed precision: FP16 cell-relative x* (sorted)
+  uint2 *xh;      // mixed precision: FP16 x* - cell origin (sorted)
+  int *kc;        // mixed precision: cell id of each sorted slot
   uint4 *nl;      // hit lists: kMaxHits/8 groups of 8 x 16-bit entries per slot
   int *ncnt;      // hits per slot (> kMaxHits: list overflowed)
   int *nbr;       // probe: neighbour counts (sorted)

@@ -8,7 +8,7 @@ benchmark configurations:
   c1  100^3 dam break, box 12 x 6 x 6, 200 steps (1,000,000 particles)
   c2  two 128 x 256 x 128 columns colliding, box 38.4 x 19.2 x 19.2
   c3  256 independent scenes, N_s log-uniform in [2048, 65536]
-  c4  256 x 128 x 128 block + seeded sphere obstacle, box 16 x 8 x 8
+  c4  256 x 128 x 128 block + seeded sphere obstacle, box 24 x 8 x 8
 
 Common: h = 0.1, lattice spacing 0.5 h, first site at min + 1.1 h, jitter
 uniform in +-0.01 h from splitmix64 (seed 1000 + config index), v = 0,
@@ -163,29 +163,38 @@ def config(index: int, physics: int = pbf.PHYSICS_PAPER, **kw) -> Scenario:
     if index == 3:
         return batch_config(physics=physics, **kw)
     if index == 4:
-        dims = (256, 128, 128)
-        box = (16.0, 8.0, 8.0)
-        pos = lattice(dims, (MARGIN, MARGIN, MARGIN), seed=1004)
-        rng = _Stream(2004)
-        r = rng.uniform(0.1, 0.3) * box[1]
-        lo = np.full(3, MARGIN) - (r + H)
-        hi = np.array([MARGIN + (d - 1) * SPACING for d in dims]) + (r + H)
-        centre = None
-        for _ in range(10000):
-            c = np.array([rng.uniform(r + H, box[a] - r - H) for a in range(3)])
-            # outside the block grown by r + h (distance to the box > r + h)
-            d = np.maximum(np.maximum(lo + (r + H) - c, c - hi + (r + H)), 0.0)
-            outside = np.any((c < lo) | (c > hi))
-            if outside and np.linalg.norm(d) > r + H:
-                centre = c
-                break
-        if centre is None:
-            raise RuntimeError("no sphere placement found")
-        sphere = (float(centre[0]), float(centre[1]), float(centre[2]), float(r))
-        params = make_params(box, physics, time_end=50 * DT, sphere=sphere, **kw)
-        return Scenario("c4", [params], [particles_from_positions(pos)], 50,
-                        {"dims": dims, "box": box, "sphere": sphere})
+        return sphere_dam((256, 128, 128), (24.0, 8.0, 8.0), seed=1004,
+                          steps=50, name="c4", physics=physics, **kw)
     raise ValueError(f"unknown config {index}")
+
+
+def sphere_dam(dims, box, seed: int, steps: int = 50, name: str = "sphere",
+               physics: int = pbf.PHYSICS_PAPER, **kw) -> Scenario:
+    """A jittered block in a box with a seeded obstacle sphere (radius
+    U[0.1, 0.3] of the box height) placed outside the block grown by r + h,
+    so the fluid falls onto or flows around it.  c4 is (256, 128, 128) in a
+    24 x 8 x 8 box (room for the obstacle beside the 12.8 x 6.4 x 6.4
+    block)."""
+    pos = lattice(dims, (MARGIN, MARGIN, MARGIN), seed=seed)
+    rng = _Stream(seed + 1000)
+    r = rng.uniform(0.1, 0.3) * box[1]
+    lo = np.full(3, MARGIN) - (r + H)
+    hi = np.array([MARGIN + (d - 1) * SPACING for d in dims]) + (r + H)
+    centre = None
+    for _ in range(10000):
+        c = np.array([rng.uniform(r + H, box[a] - r - H) for a in range(3)])
+        # outside the block grown by r + h (distance to the box > r + h)
+        d = np.maximum(np.maximum(lo + (r + H) - c, c - hi + (r + H)), 0.0)
+        outside = np.any((c < lo) | (c > hi))
+        if outside and np.linalg.norm(d) > r + H:
+            centre = c
+            break
+    if centre is None:
+        raise RuntimeError("no sphere placement found")
+    sphere = (float(centre[0]), float(centre[1]), float(centre[2]), float(r))
+    params = make_params(box, physics, time_end=steps * DT, sphere=sphere, **kw)
+    return Scenario(name, [params], [particles_from_positions(pos)], steps,
+                    {"dims": dims, "box": box, "sphere": sphere})
 
 
 def batch_config(nscenes: int = 256, physics: int = pbf.PHYSICS_PAPER,

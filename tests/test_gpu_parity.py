@@ -74,14 +74,15 @@ def test_probe_c0_ref_bit_exact(c0_ref):
 
 
 def test_probe_paper_counts_equal_brute_force():
-    """At the binned positions the 27-cell search finds every pair within h:
-    the PAPER neighbour counts are the exact all-pairs counts."""
+    """At the binned positions the row search (3 x 3 y/z cells, fine x bins
+    cut to the reach) finds every pair within h: the PAPER neighbour counts
+    are the exact all-pairs counts."""
     sc = scenarios.config(0, physics=pbf.PHYSICS_PAPER)
     say("probe c0 paper", sc.n)
     p, parts = sc.params[0], sc.particles[0]
     with pbf.Simulation(p, parts) as s:
         g = s.probe()
-        assert s.num_cells() == 40 * 20 * 20
+        assert s.num_cells() == 4 * 40 * 20 * 20   # 4 fine x bins per cell
     brute = orc.brute_neighbors(g["pred"], float(np.float32(0.1) ** 2))
     assert np.array_equal(g["nbr_count"], brute)
     assert brute.mean() > 20
@@ -187,6 +188,89 @@ def test_paper_sphere_obstacle():
     assert d.min() >= r + 0.1 - 1e-5
     d0 = np.linalg.norm(parts[:, 1:4] - np.float32(centre), axis=1)
     assert d0.min() < r + 0.1 + 0.02   # the obstacle is in play
+
+
+# ----------------------------------------------------------------------------
+# mixed precision (FP16 cell-relative gathers in the leading lambda passes)
+# ----------------------------------------------------------------------------
+
+def mixed(p, fp32_iterations=1):
+    q = p.copy()
+    q.precision, q.fp32_iterations = pbf.PRECISION_MIXED, fp32_iterations
+    return q
+
+
+# An FP16 offset inside a cell of length h = 0.1 is rounded by at most
+# 0.1 * 2^-12 = 2.4e-5 m; it enters only the lambda values of the FP16
+# passes (the position updates read FP32 positions), so after 10 steps the
+# mixed run stays within 2e-4 m (2e-3 h) of the FP64 oracle, twice the FP32
+# bar above.
+MIXED_POS_TOL = 2e-4
+
+
+def test_mixed_vs_fp64_oracle():
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=21)
+    p, parts = sc.params[0], sc.particles[0]
+    say("mixed vs fp64", sc.n)
+    with pbf.Simulation(mixed(p), parts) as s:
+        assert s.config()[3] == p.iterations - 1
+        s.step(10)
+        gm, gvm = s.dump()
+        dm = s.diagnostics()
+    with pbf.Simulation(p, parts) as s:
+        s.step(10)
+        gf, _ = s.dump()
+        df = s.diagnostics()
+    pos, vel = orc.state_from_particles(parts, np.float64)
+    op, ov = orc.run_f64(p, pos, vel, 10)
+    em = float(np.abs(gm[:, :3] - op[:, :3]).max())
+    ef = float(np.abs(gf[:, :3] - op[:, :3]).max())
+    say("max |x - x_fp64|: mixed", em, "fp32", ef)
+    assert em <= MIXED_POS_TOL
+    # v = (x* - x) / dt: the position bar over one time step
+    assert np.abs(gvm[:, :3] - ov[:, :3]).max() <= MIXED_POS_TOL / p.time_step_length
+    # density error of the mixed run against the oracle's FP64 density at
+    # the oracle's positions
+    rho = orc.density_f64(p, op)
+    ref_err = float(np.abs(rho / p.rest_density - 1.0).max())
+    say("max density err: mixed", dm["max_compression"], "fp32",
+        df["max_compression"], "fp64", ref_err)
+    assert abs(dm["max_compression"] - ref_err) <= 5e-3
+    assert abs(dm["max_compression"] - df["max_compression"]) <= 5e-3
+
+
+def test_mixed_all_fp32_iterations_equals_plain():
+    sc = scenarios.small_dam(dims=(12, 12, 12), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=5)
+    p, parts = sc.params[0], sc.particles[0]
+    say("mixed fp32_iterations == iterations", sc.n)
+    a = run_gpu(mixed(p, p.iterations), parts, 5)
+    b = run_gpu(p, parts, 5)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_mixed_sphere_and_batch():
+    centre, r = (0.385, 0.3, 0.385), 0.2
+    sc = scenarios.small_dam(dims=(12, 12, 12), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=4,
+                             origin=(0.11, 0.61, 0.11),
+                             sphere=(*centre, r))
+    p, parts = mixed(sc.params[0], 2), sc.particles[0]
+    say("mixed sphere + batch", sc.n)
+    gp, _ = run_gpu(p, parts, 10)
+    pos, vel = orc.state_from_particles(parts, np.float64)
+    op, _ = orc.run_f64(sc.params[0], pos, vel, 10)
+    assert np.abs(gp[:, :3] - op[:, :3]).max() <= MIXED_POS_TOL
+    d = np.linalg.norm(gp[:, :3] - np.float32(centre), axis=1)
+    assert d.min() >= r + 0.1 - 1e-5
+    # two copies of the scene in one batched handle: bit-identical to the
+    # single-scene mixed run
+    with pbf.BatchSimulation([p, p], [parts, parts]) as b:
+        b.step(10)
+        for k in range(2):
+            bp, _ = b.dump_scene(k)
+            assert np.array_equal(bp, gp)
 
 
 # ----------------------------------------------------------------------------

@@ -114,6 +114,23 @@ def test_create_rejects_bad_params():
     bad.x_max = bad.x_min
     with pytest.raises(pbf.PBFError):
         pbf.Simulation(bad, parts)
+    # mixed precision: PAPER physics only, 1 <= fp32_iterations <= iterations
+    bad = p.copy()
+    bad.precision, bad.fp32_iterations = pbf.PRECISION_MIXED, 1
+    with pytest.raises(pbf.PBFError) as e:
+        pbf.Simulation(bad, parts)
+    assert "PAPER" in str(e.value) or "paper" in str(e.value)
+    q = pbf.default_params(pbf.PHYSICS_PAPER)
+    for name in ("x_max", "y_max", "z_max", "x_n", "y_n", "z_n",
+                 "time_step_length", "rest_density"):
+        setattr(q, name, getattr(p, name))
+    q.precision = pbf.PRECISION_MIXED
+    for fp32_it in (0, q.iterations + 1):
+        bad = q.copy()
+        bad.fp32_iterations = fp32_it
+        with pytest.raises(pbf.PBFError) as e:
+            pbf.Simulation(bad, parts)
+        assert "fp32_iterations" in str(e.value)
 
 
 def test_scenario_configs_shapes():
