@@ -189,7 +189,9 @@ def _run_to(params, parts, steps: int, timed: int, warmup: int):
 
 def mixed_vs_plain(timed: int, oracle_threads: int, with_oracle: bool):
     """The 4,194,304-particle sphere dam (c4), PAPER physics: FP32 against
-    mixed precision (3 FP16-gather iterations + 1 FP32 iteration), both run
+    mixed precision (iteration 0 the FP32 neighbour search, the lambda
+    passes of iterations 1-2 over FP16 neighbour positions, iteration 3
+    FP32), both run
     50 steps from the same input; the speed of each over `timed` graph
     launches, the GPU FP64 density error after 50 steps, and the position
     RMS between the two.  The FP64 oracle comparison runs on a 65,536-

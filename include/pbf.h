@@ -46,9 +46,11 @@ extern "C" {
 #define PBF_PHYSICS_PAPER 1
 
 /* ---- precision --------------------------------------------------------- */
-/* PBF_PRECISION_MIXED: the first (iterations - fp32_iterations) constraint
- * iterations gather FP16 cell-relative positions (kernels evaluated and
- * accumulated in FP32); the remaining iterations are full FP32. */
+/* PBF_PRECISION_MIXED (PAPER only): the lambda passes of iterations
+ * 1 .. iterations - fp32_iterations - 1 gather FP16 bin-relative neighbour
+ * positions (8-byte records; kernels evaluated and accumulated in FP32).
+ * Iteration 0 is the FP32 neighbour search, whose lists every pass of the
+ * step uses, and the last fp32_iterations iterations are full FP32. */
 #define PBF_PRECISION_FP32 0
 #define PBF_PRECISION_MIXED 1
 
@@ -222,7 +224,7 @@ int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms);
  * 0 and the handle has at least 2^18 slots; cached per shape, and in the
  * file named by the PBF_TUNE_CACHE environment variable), cfg[1] =
  * particles per work-item (1), cfg[2] = sort passes (1: one counting pass),
- * cfg[3] = FP16 iterations per step. */
+ * cfg[3] = FP16 lambda passes per step. */
 int pbf_get_config(const pbf_sim *sim, int32_t cfg[4]);
 
 #ifdef __cplusplus

@@ -14,6 +14,9 @@
 //     compute_scaling.cl:33-88.
 //   * Real = double: the FP64 oracle of both physics modes (REF and PAPER)
 //     that gives per-result density error and position RMS.
+// REF passes walk the cell stencil of the current positions in every pass,
+// as the reference kernels do; PAPER passes sum over neighbour lists found
+// once per step at the predicted positions (Macklin & Mueller, Algorithm 1).
 //
 // Builtins the reference takes from its OpenCL implementation (sqrt,
 // length, division under -cl-fast-relaxed-math) are fixed here to IEEE
@@ -170,6 +173,7 @@ struct State {
   std::vector<R> pos, vel, pred, delta, dvel, omega;  // 4 per particle
   std::vector<R> scaling;
   std::vector<int> heads, next;
+  std::vector<std::vector<int>> nbrs;  // PAPER: N_i of the step
   R wave = 0;
 };
 
@@ -229,6 +233,39 @@ void for_candidates(const State<R> &s, int i, F &&f) {
       }
 }
 
+// PAPER neighbour lists, Algorithm 1 of Macklin & Mueller (lines 5-7):
+// N_i = {j != i : 0 < |x*_i - x*_j|^2 < h^2} at the predicted positions,
+// fixed for the step.
+template <typename R>
+void build_neighbours(State<R> &s) {
+  const auto &k = s.k;
+  s.nbrs.resize(s.n);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < s.n; ++i) {
+    std::vector<int> &l = s.nbrs[i];
+    l.clear();
+    const R *xi = &s.pred[4 * i];
+    for_candidates(s, i, [&](int j) {
+      const R *xj = &s.pred[4 * j];
+      const R d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+      const R r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 > (R)0 && r2 < k.h2) l.push_back(j);
+    });
+  }
+}
+
+// The pairs a pass sums over: REF walks the 27-cell stencil of the current
+// position (compute_scaling.cl:21-88); PAPER the step's N_i.  Each pass
+// tests 0 < r^2 < h^2 at its own positions.
+template <typename R, typename F>
+void for_pairs(const State<R> &s, int i, F &&f) {
+  if (s.k.physics == PBF_PHYSICS_PAPER) {
+    for (int j : s.nbrs[i]) f(j);
+  } else {
+    for_candidates(s, i, f);
+  }
+}
+
 // r = x_i - x_j and r^2 = r.x*r.x + r.y*r.y + r.z*r.z, left to right
 // (compute_scaling.cl:58-59).
 template <typename R>
@@ -257,7 +294,7 @@ void compute_scaling(State<R> &s) {
     const R *xi = &s.pred[4 * i];
     R density = 0, gsum = 0, gsum2 = 0;
     R gi[3] = {0, 0, 0};
-    for_candidates(s, i, [&](int j) {
+    for_pairs(s, i, [&](int j) {
       R r[3];
       const R r2 = dist2(xi, &s.pred[4 * j], r);
       if (r2 > (R)0 && r2 < k.h2) {
@@ -335,7 +372,7 @@ void compute_delta_and_update(State<R> &s) {
     const R *xi = &s.pred[4 * i];
     R sum[3] = {0, 0, 0};
     const R li = s.scaling[i];
-    for_candidates(s, i, [&](int j) {
+    for_pairs(s, i, [&](int j) {
       R r[3];
       const R r2 = dist2(xi, &s.pred[4 * j], r);
       if (r2 > (R)0 && r2 < k.h2) {
@@ -386,7 +423,7 @@ void xsph_vorticity(State<R> &s) {
     const R *vi = &s.vel[4 * i];
     R vs[3] = {0, 0, 0};
     R om[3] = {0, 0, 0};
-    for_candidates(s, i, [&](int j) {
+    for_pairs(s, i, [&](int j) {
       R r[3];
       const R *xj = &s.pred[4 * j];
       const R r2 = dist2(xi, xj, r);
@@ -424,7 +461,7 @@ void vorticity_force(State<R> &s) {
     const R *xi = &s.pred[4 * i];
     const R wi = s.omega[4 * i + 3];
     R eta[3] = {0, 0, 0};
-    for_candidates(s, i, [&](int j) {
+    for_pairs(s, i, [&](int j) {
       R r[3];
       const R r2 = dist2(xi, &s.pred[4 * j], r);
       if (r2 > (R)0 && r2 < k.h2) {
@@ -465,6 +502,7 @@ template <typename R>
 void step(State<R> &s) {
   predict(s);
   build_lists(s);
+  if (s.k.physics == PBF_PHYSICS_PAPER) build_neighbours(s);
   for (int it = 0; it < s.k.iterations; ++it) {
     compute_scaling(s);
     compute_delta_and_update(s);
@@ -579,6 +617,7 @@ int orc_probe_f32(const pbf_params *p, int n, const float *pos4,
   std::memcpy(s.vel.data(), vel4, sizeof(float) * 4 * n);
   predict(s);
   build_lists(s);
+  if (s.k.physics == PBF_PHYSICS_PAPER) build_neighbours(s);
   std::vector<long> cell(n);
   for (int i = 0; i < n; ++i) cell[i] = cell_id(s.k, &s.pred[4 * i]);
   if (cell_of)
@@ -677,6 +716,7 @@ int orc_vorticity_f64(const pbf_params *p, int n, const double *pos4,
   }
   for (int i = 0; i < n; ++i) s.pred[4 * i + 3] = 1.0;  // rho_j for XSPH
   build_lists(s);
+  build_neighbours(s);
   xsph_vorticity(s);
   std::memcpy(omega4, s.omega.data(), sizeof(double) * 4 * n);
   return 0;

@@ -320,8 +320,9 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   sh.scan_blocks = (int)((cells + pbf::kScanTile - 1) / pbf::kScanTile);
   sh.physics = p0.physics == PBF_PHYSICS_REF ? pbf::kRef : pbf::kPaper;
   sh.iterations = p0.iterations;
+  // iteration 0 searches in FP32; iterations 1 .. I - F - 1 are FP16
   sh.fp16_iterations = p0.precision == PBF_PRECISION_MIXED
-                           ? p0.iterations - p0.fp32_iterations
+                           ? std::max(p0.iterations - p0.fp32_iterations - 1, 0)
                            : 0;
   if (sh.fp16_iterations > 0)
     for (const pbf::Scene &sc : S.scenes)
@@ -351,6 +352,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   d.pid_s = S.alloc<int>(P);
   d.xa = S.alloc<float4>(P);
   d.xb = S.alloc<float4>(P);
+  if (sh.physics == pbf::kPaper) d.xc = S.alloc<float4>(P);
   d.xold = S.alloc<float4>(P);
   d.lam = S.alloc<float>(P);
   d.rho = S.alloc<float>(P);
@@ -363,8 +365,10 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     d.xh = S.alloc<uint2>(P);
     d.kc = S.alloc<int>(P);
   }
-  d.nl = S.alloc<uint4>(P * (size_t)(pbf::kMaxHits / 8));
-  d.ncnt = S.alloc<int>(P);
+  if (sh.physics == pbf::kPaper) {
+    d.nl = S.alloc<uint4>(P * (size_t)(pbf::kMaxHits / 8));
+    d.ncnt = S.alloc<int>(P);
+  }
   d.nbr = S.alloc<int>(P);
   d.red = S.alloc<double>((P / 64 + 1) * 8);
   d.err = S.alloc<int>(1);

@@ -529,7 +529,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     const int *__restrict__ key, const int *__restrict__ cell_start,
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
     int *__restrict__ pid_s, float4 *__restrict__ xa,
-    float4 *__restrict__ xold, int *__restrict__ kc, uint2 *__restrict__ xh) {
+    float4 *__restrict__ xold, int *__restrict__ kc) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int t = lb * blockDim.x + threadIdx.x;
@@ -546,28 +546,28 @@ __global__ __launch_bounds__(256) void k_reorder(
   pid_s[f] = p;
   xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
   xold[f] = make_float4(x.x, x.y, x.z, pu.w);
-  if (xh) {  // mixed precision: sorted cell id and FP16 cell-relative x*
-    kc[f] = c;
-    xh[f] = pack_rel(sc, make_float3(pu.x, pu.y, pu.z), c);
-  }
+  if (kc) kc[f] = c;  // mixed precision: the bin of each sorted slot
 }
 
 // ---------------------------------------------------------------------------
-// hit lists (PAPER)
+// neighbour lists (PAPER)
 //
-// The lambda pass of an iteration and the delta pass that follows it see the
-// same positions, so they have the same neighbours (0 < r^2 < h^2); so do the
-// XSPH pass and the vorticity pass.  The searching pass records its hits in
-// visiting order and the following pass replays them: the same sums in the
-// same order, without walking the 27 cells again.
+// PAPER follows Algorithm 1 of Macklin & Mueller: the neighbours of a
+// particle are found once per step, at the predicted positions x*,
+// N_i = {j != i : 0 < |x*_i - x*_j|^2 < h^2}, and every pass of the step
+// (lambda and delta of each iteration, XSPH, vorticity) sums over N_i at its
+// own positions, a pair contributing while 0 < r < h.  The first lambda pass
+// searches and records N_i in visiting order; the other passes replay it.
 //
 // An entry is 16 bits: stencil row (dz, dy) in the top 4 bits, offset of the
-// slot from the start of that row's slot range in the low 12.  Entries are
+// slot from the start of that row's slot range in the low 12 (rows and row
+// starts are those of x*_i, recomputed by the replaying pass).  Entries are
 // packed eight to a 16-byte group in registers (a 128-bit shift chain) and
 // each full group is one store; group g of slot i is at nl[g * P + i], so
 // lanes store and load adjacent groups.  A particle with more than kMaxHits
-// hits (or a row longer than 4096 slots) is marked with a count of
-// kMaxHits + 1 and searched again by the replaying pass.
+// neighbours (or a row longer than 4096 slots) is marked with a count of
+// kMaxHits + 1; the replaying passes then walk its rows again with the
+// search's test at x*.
 // ---------------------------------------------------------------------------
 
 struct HitRecorder {
@@ -641,51 +641,53 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
   }
 }
 
-// Replay of the recorded hits of slot i: f(j, X[j], true) in recorded
-// order, eight entries resolved and their eight loads issued before any is
-// used.  An overflowed list walks the stencil instead: f(j, X[j], false),
-// the caller testing r^2.  rb is LDS scratch of 9 x blockDim.x ints.
-template <typename F>
-__device__ __forceinline__ void replay_hits(const Scene &sc,
-                                            const int *__restrict__ cs,
-                                            const float4 *__restrict__ X,
-                                            const uint4 *__restrict__ nl,
-                                            const int *__restrict__ ncnt,
-                                            int stride, int i, float3 xi,
-                                            int *rb, F &&f) {
+// f(j, row, X[j]) for every j of N_i in recorded order, eight entries
+// resolved and their eight loads issued before any is used.  XS = x* (the
+// rows), xs = x*_i; rb is LDS scratch of 9 x blockDim.x ints.
+template <typename T, typename F>
+__device__ __forceinline__ void for_neighbours(
+    const Scene &sc, const int *__restrict__ cs, const float4 *__restrict__ XS,
+    const T *__restrict__ X, const uint4 *__restrict__ nl,
+    const int *__restrict__ ncnt, int stride, int i, float3 xs, int *rb,
+    F &&f) {
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
     const int lane = threadIdx.x, bs = blockDim.x;
     int cx, cy, cz;
-    cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
+    cell_coords<kPaper>(sc, xs.x, xs.y, xs.z, cx, cy, cz);
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
       const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
       int start = 0, end = 0;
       if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1])
-        row_range(sc, cs, xi, yy, zz, start, end);
+        row_range(sc, cs, xs, yy, zz, start, end);
       rb[r * bs + lane] = start;
     }
     for (int g = 0; g * 8 < cnt; ++g) {
       const uint4 w = nl[(size_t)g * stride + i];
       const unsigned ws[4] = {w.x, w.y, w.z, w.w};
-      int js[8];
+      int js[8], rows[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const unsigned e = (ws[q >> 1] >> ((q & 1) * 16)) & 0xffffu;
-        js[q] = (g * 8 + q < cnt) ? rb[(e >> 12) * bs + lane] + (int)(e & 0xfffu)
+        rows[q] = (int)(e >> 12);
+        js[q] = (g * 8 + q < cnt) ? rb[rows[q] * bs + lane] + (int)(e & 0xfffu)
                                   : i;
       }
-      float4 xs[8];
+      T xv[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) xs[q] = X[js[q]];
+      for (int q = 0; q < 8; ++q) xv[q] = X[js[q]];
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        if (g * 8 + q < cnt) f(js[q], xs[q], true);
+        if (g * 8 + q < cnt) f(js[q], rows[q], xv[q]);
     }
   } else {
-    walk_rows(sc, cs, X, xi,
-              [&](int j, float4 xj, int, int) { f(j, xj, false); });
+    const float h2 = sc.h2;
+    walk_rows(sc, cs, XS, xs, [&](int j, float4 sj, int row, int) {
+      const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 > 0.0f && r2 < h2) f(j, row, X[j]);
+    });
   }
 }
 
@@ -785,163 +787,167 @@ __global__ __launch_bounds__(256) void k_xsph_final_ref(
 // ---------------------------------------------------------------------------
 // constraint solve, PAPER
 //
-// Buffers: A holds x* (after the last delta pass .w = rho); the lambda pass
-// writes B = (x*, lambda) so that the delta pass reads one float4 per
-// neighbour, and writes the new x* back into A.
+// Buffers: XS = x* (sorted; it fixes the bins, rows and neighbour lists of
+// the step and is not written after the sort), B = (x, lambda) written by a
+// lambda pass so that the delta pass reads one float4 per neighbour, and
+// XC = x after a delta pass (.w = rho after the last one).  Iteration k's
+// lambda pass reads XS (k = 0, the search) or XC; its delta pass reads B and
+// writes XC.
 // ---------------------------------------------------------------------------
 
-// eqs. 1, 2, 8, 9, 11; records the hit list for the delta pass.
+// Sums of (h^2-r^2)^3, (h-r)^4 and (h-r)^2 r/|r| over the pairs of one
+// particle, constants applied once: density and lambda (eqs. 1, 2, 8, 9,
+// 11).
+struct LambdaSums {
+  float wsum = 0.0f, g2 = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  __device__ __forceinline__ void add(float h, float h2, float rx, float ry,
+                                      float rz, float r2) {
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float hr = h - r2 * inv;
+    const float t = h2 - r2;
+    const float hr2 = hr * hr;
+    wsum += t * t * t;
+    g2 += hr2 * hr2;
+    const float gs = hr2 * inv;
+    gx += rx * gs;
+    gy += ry * gs;
+    gz += rz * gs;
+  }
+  __device__ __forceinline__ void store(const Scene &sc, int i, float3 xi,
+                                        float4 *__restrict__ B,
+                                        float *__restrict__ rho) const {
+    const float dens = sc.poly6 * wsum;
+    const float sp2 = sc.spiky * sc.spiky;
+    const float denom = sp2 * (gx * gx + gy * gy + gz * gz + g2) *
+                            (sc.inv_rho0 * sc.inv_rho0) +
+                        sc.eps;
+    B[i] = make_float4(xi.x, xi.y, xi.z, -(dens * sc.inv_rho0 - 1.0f) / denom);
+    rho[i] = dens;
+  }
+};
+
+// Iteration 0: searches the rows of x*_i, records N_i, and forms lambda.
 __global__ __launch_bounds__(256) void k_lambda_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ A, const int *__restrict__ cs,
+    const float4 *__restrict__ XS, const int *__restrict__ cs,
     float4 *__restrict__ B, float *__restrict__ rho, uint4 *__restrict__ nl,
     int *__restrict__ ncnt, int stride) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = A[i];
+  const float4 x4 = XS[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
-  // sums of (h^2-r^2)^3, (h-r)^4 and (h-r)^2 r/|r|; constants applied once
-  float wsum = 0.0f, g2 = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
-  walk_rows(sc, cs, A, xi, [&](int j, float4 xj, int row, int b) {
+  walk_rows(sc, cs, XS, xi, [&](int j, float4 xj, int row, int b) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
       rec.add(j, row, b, nl, stride, i);
-      const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
-      const float t = h2 - r2;
-      const float hr2 = hr * hr;
-      wsum += t * t * t;
-      g2 += hr2 * hr2;
-      const float gs = hr2 * inv;
-      gx += rx * gs;
-      gy += ry * gs;
-      gz += rz * gs;
+      ls.add(h, h2, rx, ry, rz, r2);
     }
   });
   rec.flush(nl, ncnt, stride, i);
-  const float dens = sc.poly6 * wsum;
-  const float sp2 = sc.spiky * sc.spiky;
-  const float denom =
-      sp2 * (gx * gx + gy * gy + gz * gz + g2) * (sc.inv_rho0 * sc.inv_rho0) +
-      sc.eps;
-  B[i] = make_float4(xi.x, xi.y, xi.z, -(dens * sc.inv_rho0 - 1.0f) / denom);
-  rho[i] = dens;
+  ls.store(sc, i, xi, B, rho);
 }
 
-// Mixed precision: the same pass over FP16 bin-relative positions: the rows
-// and the recorded hits are those of the FP32 pass; a candidate's bin origin
-// is its row's y/z corner and its fine x index times the fine cell length:
-// r = (x_i - origin) - offset_j, evaluated and summed in FP32.
-__global__ __launch_bounds__(256) void k_lambda_paper_h(
+// Iterations >= 1: lambda over N_i at the positions of the last delta pass.
+__global__ __launch_bounds__(256) void k_lambda_list(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ A, const uint2 *__restrict__ XH,
-    const int *__restrict__ cs, float4 *__restrict__ B,
-    float *__restrict__ rho, uint4 *__restrict__ nl, int *__restrict__ ncnt,
-    int stride) {
+    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
+    const int *__restrict__ cs, const uint4 *__restrict__ nl,
+    const int *__restrict__ ncnt, int stride, float4 *__restrict__ B,
+    float *__restrict__ rho) {
+  extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = A[i];
+  const float4 s4 = XS[i], x4 = XC[i];
+  const float3 xs = make_float3(s4.x, s4.y, s4.z);
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
-  float wsum = 0.0f, g2 = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
-  // The self pair is skipped by index: its FP16 offset differs from the FP32
-  // one by the rounding, so r^2 > 0 would admit it, and the delta pass
-  // replaying the hit at r = 0 would divide by zero.
-  auto pair = [&](int j, float rx, float ry, float rz, int row, int b) {
+  for_neighbours(sc, cs, XS, XC, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int, int, float4 xj) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (j != i && r2 > 0.0f && r2 < h2) {
-      rec.add(j, row, b, nl, stride, i);
-      const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
-      const float t = h2 - r2;
-      const float hr2 = hr * hr;
-      wsum += t * t * t;
-      g2 += hr2 * hr2;
-      const float gs = hr2 * inv;
-      gx += rx * gs;
-      gy += ry * gs;
-      gz += rz * gs;
-    }
-  };
-  int cx, cy, cz;
-  cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
-  const float ux = xi.x - sc.bmin[0], wx = sc.cell_len[0];
-  auto cand = [&](int j, uint2 w, float qy, float qz, int row, int b) {
-    const float ox = (float)(w.y >> 16) * wx;
-    pair(j, (ux - ox) - half_lo(w.x), qy - half_hi(w.x), qz - half_lo(w.y),
-         row, b);
-  };
-  for (int dz = 0; dz < 3; ++dz) {
-    const int zz = cz - 1 + dz;
-    if (zz < 0 || zz >= sc.nc[2]) continue;
-    const float qz = xi.z - (sc.bmin[2] + zz * sc.cell_len[2]);
-    for (int dy = 0; dy < 3; ++dy) {
-      const int yy = cy - 1 + dy;
-      if (yy < 0 || yy >= sc.nc[1]) continue;
-      const float qy = xi.y - (sc.bmin[1] + yy * sc.cell_len[1]);
-      int b, e;
-      row_range(sc, cs, xi, yy, zz, b, e);
-      const int row = dz * 3 + dy;
-      int j = b;
-      for (; j + 4 <= e; j += 4) {
-        const uint2 w0 = XH[j], w1 = XH[j + 1], w2 = XH[j + 2], w3 = XH[j + 3];
-        cand(j, w0, qy, qz, row, b);
-        cand(j + 1, w1, qy, qz, row, b);
-        cand(j + 2, w2, qy, qz, row, b);
-        cand(j + 3, w3, qy, qz, row, b);
-      }
-      for (; j < e; ++j) cand(j, XH[j], qy, qz, row, b);
-    }
-  }
-  rec.flush(nl, ncnt, stride, i);
-  const float dens = sc.poly6 * wsum;
-  const float sp2 = sc.spiky * sc.spiky;
-  const float denom =
-      sp2 * (gx * gx + gy * gy + gz * gz + g2) * (sc.inv_rho0 * sc.inv_rho0) +
-      sc.eps;
-  B[i] = make_float4(xi.x, xi.y, xi.z, -(dens * sc.inv_rho0 - 1.0f) / denom);
-  rho[i] = dens;
+    if (r2 > 0.0f && r2 < h2) ls.add(h, h2, rx, ry, rz, r2);
+  });
+  ls.store(sc, i, xi, B, rho);
 }
 
-// eqs. 13, 14 and the collision clamp over the hits of the lambda pass.
-// LAST also forms v = (x* - x)/dt and carries rho in .w for XSPH.
-// WRITE_H (mixed precision) also refreshes the FP16 cell-relative copy for
-// the next FP16 lambda pass.
+// Mixed precision, iterations >= 1: k_lambda_list over FP16 bin-relative
+// neighbour positions (XH, written by the delta pass before).  The entry's
+// row gives the y/z bin corner (rows of x*_i), the record the fine x index.
+__global__ __launch_bounds__(256) void k_lambda_list_h(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
+    const uint2 *__restrict__ XH, const int *__restrict__ cs,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
+    float4 *__restrict__ B, float *__restrict__ rho) {
+  extern __shared__ int rb_lds[];
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  if (i - sc.slot_off >= sc.n) return;
+  const float4 s4 = XS[i], x4 = XC[i];
+  const float3 xs = make_float3(s4.x, s4.y, s4.z);
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  int cx, cy, cz;
+  cell_coords<kPaper>(sc, xs.x, xs.y, xs.z, cx, cy, cz);
+  // x_i relative to the corner of the lowest stencil row (dy = dz = 0)
+  const float qx = xi.x - sc.bmin[0], wx = sc.cell_len[0];
+  const float qy0 = xi.y - (sc.bmin[1] + (cy - 1) * sc.cell_len[1]);
+  const float qz0 = xi.z - (sc.bmin[2] + (cz - 1) * sc.cell_len[2]);
+  LambdaSums ls;
+  const float h = sc.h, h2 = sc.h2;
+  for_neighbours(sc, cs, XS, XH, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int j, int row, uint2 w) {
+    const int dz = (row * 11) >> 5;  // row / 3 for row < 9
+    const int dy = row - 3 * dz;
+    const float ox = (float)(w.y >> 16) * wx;
+    const float rx = (qx - ox) - half_lo(w.x);
+    const float ry = (qy0 - (float)dy * sc.cell_len[1]) - half_hi(w.x);
+    const float rz = (qz0 - (float)dz * sc.cell_len[2]) - half_lo(w.y);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (j != i && r2 > 0.0f && r2 < h2) ls.add(h, h2, rx, ry, rz, r2);
+  });
+  ls.store(sc, i, xi, B, rho);
+}
+
+// eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST also forms
+// v = (x - x_n)/dt and carries rho in .w for XSPH.  WRITE_H (mixed
+// precision) also writes the FP16 bin-relative copy for the next FP16 lambda
+// pass.
 template <bool LAST, bool WRITE_H>
 __global__ __launch_bounds__(256) void k_delta_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ B, const int *__restrict__ cs,
-    const float *__restrict__ rho, const float4 *__restrict__ xold,
-    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
-    const int *__restrict__ kc, float4 *__restrict__ A, float4 *__restrict__ V,
+    const float4 *__restrict__ XS, const float4 *__restrict__ B,
+    const int *__restrict__ cs, const float *__restrict__ rho,
+    const float4 *__restrict__ xold, const uint4 *__restrict__ nl,
+    const int *__restrict__ ncnt, int stride, const int *__restrict__ kc,
+    float4 *__restrict__ XC, float4 *__restrict__ V,
     uint2 *__restrict__ XH) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = B[i];
+  const float4 s4 = XS[i], x4 = B[i];
+  const float3 xs = make_float3(s4.x, s4.y, s4.z);
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float li = x4.w;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
   const int n = sc.scorr_n;
-  // r^2 > 0 is tested for recorded hits too: an FP16 lambda pass can record
-  // a pair whose FP32 positions coincide (two particles clamped onto the
-  // same wall corner, binned in different cells), whose gradient is zero.
-  replay_hits(sc, cs, B, nl, ncnt, stride, i, xi, rb_lds,
-              [&](int, float4 xj, bool hit) {
+  for_neighbours(sc, cs, XS, B, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && (hit || r2 < h2)) {
+    if (r2 > 0.0f && r2 < h2) {
       const float inv = __builtin_amdgcn_rsqf(r2);
       const float hr = h - r2 * inv;
       const float t = h2 - r2;
@@ -959,7 +965,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   const float c = -sc.spiky * sc.inv_rho0;
   const float3 dp = make_float3(ax * c, ay * c, az * c);
   const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
-  A[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
+  XC[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
   if constexpr (WRITE_H) XH[i] = pack_rel(sc, xn, kc[i]);
   if constexpr (LAST) {
     const float4 xo = xold[i];
@@ -968,31 +974,35 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   }
 }
 
-// XSPH with 1/rho_j weights fused with update_positions.cl (PAPER without
-// vorticity).
+// XSPH over N_i with 1/rho_j weights fused with update_positions.cl (PAPER
+// without vorticity).
 __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ A, const int *__restrict__ cs,
-    const float4 *__restrict__ V, const int *__restrict__ pid_s,
-    float4 *__restrict__ pos, float4 *__restrict__ vel,
-    int *__restrict__ pid, int *__restrict__ err) {
+    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
+    const int *__restrict__ cs, const float4 *__restrict__ V,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
+    const int *__restrict__ pid_s, float4 *__restrict__ pos,
+    float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err) {
+  extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = A[i];
+  const float4 s4 = XS[i], x4 = XC[i];
+  const float3 xs = make_float3(s4.x, s4.y, s4.z);
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 vi = V[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
   const float h2 = sc.h2;
-  walk_rows(sc, cs, A, xi, [&](int j, float4 xj, int, int) {
+  for_neighbours(sc, cs, XS, XC, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int j, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
       const float4 vj = V[j];
       const float t = h2 - r2;
-      // a neighbour with no density yet (entered the support after the
-      // last lambda pass) carries no weight
+      // a neighbour with no density (no pair within h in the last lambda
+      // pass) carries no weight
       const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
       sx += (vj.x - vi.x) * w;
       sy += (vj.y - vi.y) * w;
@@ -1005,29 +1015,31 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
   write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
 }
 
-// XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15);
-// writes B = (x*, |omega|) and records the hit list for the vorticity pass.
+// XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15) over
+// N_i; writes B = (x, |omega|) for the vorticity pass.
 __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ A, const int *__restrict__ cs,
-    const float4 *__restrict__ V, float4 *__restrict__ VX,
-    float4 *__restrict__ OM, float4 *__restrict__ B, uint4 *__restrict__ nl,
-    int *__restrict__ ncnt, int stride) {
+    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
+    const int *__restrict__ cs, const float4 *__restrict__ V,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
+    float4 *__restrict__ VX, float4 *__restrict__ OM,
+    float4 *__restrict__ B) {
+  extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = A[i];
+  const float4 s4 = XS[i], x4 = XC[i];
+  const float3 xs = make_float3(s4.x, s4.y, s4.z);
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 vi = V[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
   const float h = sc.h, h2 = sc.h2;
-  HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
-  walk_rows(sc, cs, A, xi, [&](int j, float4 xj, int row, int b) {
+  for_neighbours(sc, cs, XS, XC, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int j, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
-      rec.add(j, row, b, nl, stride, i);
       const float4 vj = V[j];
       const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
       const float t = h2 - r2;
@@ -1044,7 +1056,6 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
       oz += vx * gy - vy * gx;
     }
   });
-  rec.flush(nl, ncnt, stride, i);
   const float cx = sc.c_xsph * sc.poly6;
   VX[i] = make_float4(vi.x + cx * sx, vi.y + cx * sy, vi.z + cx * sz, vi.w);
   ox *= sc.spiky;
@@ -1055,31 +1066,33 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
   B[i] = make_float4(xi.x, xi.y, xi.z, wl);
 }
 
-// eta = sum (|w_j| - |w_i|) grad_i W_ij, N = eta/|eta|, v += dt eps
-// (N x w_i) (eq. 16), then update_positions.cl; replays the XSPH hits over
-// B = (x*, |omega|).
+// eta = sum (|w_j| - |w_i|) grad_i W_ij over N_i, N = eta/|eta|,
+// v += dt eps (N x w_i) (eq. 16), then update_positions.cl; reads
+// B = (x, |omega|).
 __global__ __launch_bounds__(256) void k_vort_final(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ B, const int *__restrict__ cs,
-    const float4 *__restrict__ VX, const float4 *__restrict__ OM,
-    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
-    const int *__restrict__ pid_s, float4 *__restrict__ pos,
-    float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err) {
+    const float4 *__restrict__ XS, const float4 *__restrict__ B,
+    const int *__restrict__ cs, const float4 *__restrict__ VX,
+    const float4 *__restrict__ OM, const uint4 *__restrict__ nl,
+    const int *__restrict__ ncnt, int stride, const int *__restrict__ pid_s,
+    float4 *__restrict__ pos, float4 *__restrict__ vel, int *__restrict__ pid,
+    int *__restrict__ err) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = B[i];
+  const float4 s4 = XS[i], x4 = B[i];
+  const float3 xs = make_float3(s4.x, s4.y, s4.z);
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 wi = OM[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   const float h = sc.h, h2 = sc.h2;
-  replay_hits(sc, cs, B, nl, ncnt, stride, i, xi, rb_lds,
-              [&](int, float4 xj, bool hit) {
+  for_neighbours(sc, cs, XS, B, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (hit || (r2 > 0.0f && r2 < h2)) {
+    if (r2 > 0.0f && r2 < h2) {
       const float inv = __builtin_amdgcn_rsqf(r2);
       const float hr = h - r2 * inv;
       const float gs = (xj.w - wi.w) * hr * hr * inv;
@@ -1278,7 +1291,7 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
              d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
              d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
-             d.xa, d.xold, d.kc, sh.fp16_iterations > 0 ? d.xh : nullptr);
+             d.xa, d.xold, sh.fp16_iterations > 0 ? d.kc : nullptr);
 }
 
 static void step_ref(const DevBuffers &d, Launcher &L) {
@@ -1311,44 +1324,53 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
   const int grid = sh.slots / bs;
   sort_stage<kPaper>(d, L, 1);
   const size_t rb_smem = (size_t)9 * bs * sizeof(int);  // replay row starts
+  const float4 *XS = d.xa;
+  // iteration 0's lambda pass is the FP32 search; mixed precision runs the
+  // lambda passes of iterations 1 .. fp16_iterations over FP16 positions
+  auto fp16 = [&](int it) { return it >= 1 && it <= sh.fp16_iterations; };
   for (int it = 0; it < sh.iterations; ++it) {
-    if (it < sh.fp16_iterations) {
-      PBF_LAUNCH(L, kClsLambda, k_lambda_paper_h, grid, bs, d.scenes,
-                 d.block_scene, d.xa, d.xh, d.cell_start, d.xb, d.rho, d.nl,
-                 d.ncnt, sh.slots);
-    } else {
+    if (it == 0) {
       PBF_LAUNCH(L, kClsLambda, k_lambda_paper, grid, bs, d.scenes,
-                 d.block_scene, d.xa, d.cell_start, d.xb, d.rho, d.nl, d.ncnt,
+                 d.block_scene, XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt,
                  sh.slots);
+    } else if (fp16(it)) {
+      PBF_LAUNCH_SMEM(L, kClsLambda, k_lambda_list_h, grid, bs, rb_smem,
+                      d.scenes, d.block_scene, XS, d.xc, d.xh, d.cell_start,
+                      d.nl, d.ncnt, sh.slots, d.xb, d.rho);
+    } else {
+      PBF_LAUNCH_SMEM(L, kClsLambda, k_lambda_list, grid, bs, rb_smem,
+                      d.scenes, d.block_scene, XS, d.xc, d.cell_start, d.nl,
+                      d.ncnt, sh.slots, d.xb, d.rho);
     }
     if (it == sh.iterations - 1) {
       PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<true, false>), grid, bs,
-                      rb_smem, d.scenes, d.block_scene, d.xb, d.cell_start,
-                      d.rho, d.xold, d.nl, d.ncnt, sh.slots, d.kc, d.xa, d.v,
-                      d.xh);
-    } else if (it + 1 < sh.fp16_iterations) {
+                      rb_smem, d.scenes, d.block_scene, XS, d.xb,
+                      d.cell_start, d.rho, d.xold, d.nl, d.ncnt, sh.slots,
+                      d.kc, d.xc, d.v, d.xh);
+    } else if (fp16(it + 1)) {
       PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<false, true>), grid, bs,
-                      rb_smem, d.scenes, d.block_scene, d.xb, d.cell_start,
-                      d.rho, d.xold, d.nl, d.ncnt, sh.slots, d.kc, d.xa, d.v,
-                      d.xh);
+                      rb_smem, d.scenes, d.block_scene, XS, d.xb,
+                      d.cell_start, d.rho, d.xold, d.nl, d.ncnt, sh.slots,
+                      d.kc, d.xc, d.v, d.xh);
     } else {
       PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<false, false>), grid, bs,
-                      rb_smem, d.scenes, d.block_scene, d.xb, d.cell_start,
-                      d.rho, d.xold, d.nl, d.ncnt, sh.slots, d.kc, d.xa, d.v,
-                      d.xh);
+                      rb_smem, d.scenes, d.block_scene, XS, d.xb,
+                      d.cell_start, d.rho, d.xold, d.nl, d.ncnt, sh.slots,
+                      d.kc, d.xc, d.v, d.xh);
     }
   }
   if (sh.vorticity) {
-    PBF_LAUNCH(L, kClsXsph, k_xsph_omega, grid, bs, d.scenes, d.block_scene,
-               d.xa, d.cell_start, d.v, d.vx, d.om, d.xb, d.nl, d.ncnt,
-               sh.slots);
+    PBF_LAUNCH_SMEM(L, kClsXsph, k_xsph_omega, grid, bs, rb_smem, d.scenes,
+                    d.block_scene, XS, d.xc, d.cell_start, d.v, d.nl, d.ncnt,
+                    sh.slots, d.vx, d.om, d.xb);
     PBF_LAUNCH_SMEM(L, kClsVort, k_vort_final, grid, bs, rb_smem, d.scenes,
-                    d.block_scene, d.xb, d.cell_start, d.vx, d.om, d.nl,
+                    d.block_scene, XS, d.xb, d.cell_start, d.vx, d.om, d.nl,
                     d.ncnt, sh.slots, d.pid_s, d.pos, d.vel, d.pid, d.err);
   } else {
-    PBF_LAUNCH(L, kClsXsph, k_xsph_final_paper, grid, bs, d.scenes,
-               d.block_scene, d.xa, d.cell_start, d.v, d.pid_s, d.pos, d.vel,
-               d.pid, d.err);
+    PBF_LAUNCH_SMEM(L, kClsXsph, k_xsph_final_paper, grid, bs, rb_smem,
+                    d.scenes, d.block_scene, XS, d.xc, d.cell_start, d.v,
+                    d.nl, d.ncnt, sh.slots, d.pid_s, d.pos, d.vel, d.pid,
+                    d.err);
   }
 }
 

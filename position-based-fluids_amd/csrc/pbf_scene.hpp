@@ -17,12 +17,12 @@ constexpr int kSlotAlign = 256;
 // candidate range of a stencil row can be cut to the particle's reach.
 constexpr int kXSub = 4;
 
-// PAPER search reach: (1 + kReachMargin) h.  The margin absorbs the motion of
-// particles between the step's sort (which fixes the bins) and a later pass.
-constexpr float kReachMargin = 0.04f;
+// PAPER search reach: (1 + kReachMargin) h, a guard against rounding in the
+// row cut (the search runs at the positions the bins were built from).
+constexpr float kReachMargin = 1e-3f;
 
-// Capacity of a particle's hit list (neighbours within h).
-constexpr int kMaxHits = 64;
+// Capacity of a particle's neighbour list (pairs within h at x*).
+constexpr int kMaxHits = 96;
 static_assert(kMaxHits % 8 == 0, "hit lists are stored in groups of 8");
 
 // Per-scene constants, resident in device memory and read by the kernels
@@ -69,14 +69,16 @@ struct DevBuffers {
   int *perm;      // slot -> previous index, arrival order
   int *ppid;      // slot -> particle id, arrival order
   int *pid_s;     // sorted particle id
-  float4 *xa, *xb;// x* ping-pong (sorted); .w = rho after the last pass
+  float4 *xa;     // x* (sorted): bins, rows and neighbour lists of a step
+  float4 *xb;     // (x, lambda) of a lambda pass (sorted)
+  float4 *xc;     // PAPER: x after a delta pass, .w = rho after the last
   float4 *xold;   // x at step start, m (sorted)
   float *lam;     // lambda (sorted)
   float *rho;     // density of the last lambda pass (sorted)
   float4 *v;      // velocity after the constraint solve (sorted), .w = m
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
-  uint2 *xh;      // mixed precision: FP16 x* - cell origin (sorted)
+  uint2 *xh;      // mixed precision: FP16 x - bin origin (sorted)
   int *kc;        // mixed precision: cell id of each sorted slot
   uint4 *nl;      // hit lists: kMaxHits/8 groups of 8 x 16-bit entries per slot
   int *ncnt;      // hits per slot (> kMaxHits: list overflowed)
