@@ -25,27 +25,55 @@ import scenarios  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (compulsory) HBM bytes per particle per launch of each kernel
-# class, PAPER physics, FP32; "delta" is the mean over the I launches (the
-# last one also reads rho and x and writes v).
-def algorithmic_bytes(cls: str, iterations: int, physics: int) -> float:
+# class, FP32.  PAPER passes stream a neighbour list of L = ceil(n/8) 16-byte
+# groups (n neighbours, 2 bytes each); "delta" is the mean over the I
+# launches (the last one also reads rho and x_n and writes v).
+def list_bytes(counts) -> float:
+    """Mean list bytes per particle for neighbour counts `counts` (lists
+    longer than 96 entries are not stored)."""
+    c = np.asarray(counts, np.float64)
+    return float(np.mean(np.where(c <= 96, 16.0 * np.ceil(c / 8.0), 0.0)))
+
+
+def algorithmic_bytes(cls: str, iterations: int, physics: int,
+                      lb: float = 80.0) -> float:
     last_extra = 36.0 / iterations
-    table = {
-        "predict_count": 56.0,
-        "scatter": 24.0,
-        "reorder": 88.0,
-        "lambda": 24.0,
-        "delta": 36.0 + last_extra,
-        "xsph": 64.0 if physics == pbf.PHYSICS_PAPER else 84.0,
-        "vorticity": 88.0,
-    }
+    if physics == pbf.PHYSICS_PAPER:
+        table = {
+            "predict_count": 56.0,      # x, v in; x*, key, rank out
+            "scatter": 24.0,
+            "reorder": 88.0,
+            "search": 40.0 + lb,        # x* in; (x*, lambda), rho, count, list out
+            "lambda": 56.0 + lb,        # x*, x, count, list in; (x, lambda), rho out
+            "delta": 52.0 + lb + last_extra,  # x*, (x, lambda), count, list in; x out
+            "xsph": 116.0 + lb,         # x*, x, v, count, list in; v', omega, (x, |w|) out
+            "vorticity": 128.0 + lb,    # x*, (x,|w|), omega, v', count, list, id in; x, v, id out
+        }
+    else:
+        table = {
+            "predict_count": 56.0, "scatter": 24.0, "reorder": 88.0,
+            "lambda": 24.0, "delta": 36.0 + last_extra, "xsph": 84.0,
+        }
     return table.get(cls, 0.0)
 
 
-def step_bytes(n: int, cells: int, iterations: int, physics: int) -> float:
-    """Compulsory bytes per step: 324 + 60 I + 16 C/N per particle (PAPER),
-    244 + 60 I + 16 C/N (REF)."""
-    base = 324.0 if physics == pbf.PHYSICS_PAPER else 244.0
-    return n * (base + 60.0 * iterations) + 16.0 * cells
+def step_bytes(n: int, cells: int, iterations: int, physics: int,
+               lb: float = 80.0) -> float:
+    """Compulsory bytes of one step: the sum of the per-launch figures above
+    plus 16 bytes per cell (count, scan, table)."""
+    per = sum(algorithmic_bytes(c, iterations, physics, lb)
+              for c in ("predict_count", "scatter", "reorder"))
+    if physics == pbf.PHYSICS_PAPER:
+        per += (algorithmic_bytes("search", iterations, physics, lb)
+                + (iterations - 1) * algorithmic_bytes("lambda", iterations, physics, lb)
+                + iterations * algorithmic_bytes("delta", iterations, physics, lb)
+                + algorithmic_bytes("xsph", iterations, physics, lb)
+                + algorithmic_bytes("vorticity", iterations, physics, lb))
+    else:
+        per += iterations * (algorithmic_bytes("lambda", iterations, physics)
+                             + algorithmic_bytes("delta", iterations, physics))
+        per += algorithmic_bytes("xsph", iterations, physics)
+    return n * per + 16.0 * cells
 
 
 def load_traffic(workload: str):
@@ -317,18 +345,20 @@ def main():
     diag = sim.diagnostics()
     prof = sim.profile(args.profile_steps)
     cfg = sim.config()
+    # neighbour counts at the current state give the mean list length
+    lb = list_bytes(sim.probe()["nbr_count"])
     sim.close()
 
     total_prof = sum(v[0] for v in prof.values())
     dom = max(prof, key=lambda k: prof[k][0])
     dom_ms, dom_launches = prof[dom]
     avg_ms = dom_ms / max(dom_launches, 1)
-    dom_bytes = algorithmic_bytes(dom, iters, physics) * n
+    dom_bytes = algorithmic_bytes(dom, iters, physics, lb) * n
     achieved = dom_bytes / (avg_ms / 1e3) / 1e9
     workload = f"c{args.config}"
     traffic = load_traffic(workload)
 
-    b_step = step_bytes(n, cells, iters, physics)
+    b_step = step_bytes(n, cells, iters, physics, lb)
     step_gbs = b_step * args.steps / (ms / 1e3) / 1e9
 
     out = {
@@ -365,6 +395,7 @@ def main():
             "algorithmic_bytes_per_launch": dom_bytes,
         },
         "step_roofline": {
+            "list_bytes_per_particle": lb,
             "bytes_per_step": b_step,
             "achieved_GBps": step_gbs,
             "frac": step_gbs / HBM_PEAK_GBS,

@@ -313,6 +313,9 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     S.scenes.push_back(sc);
   }
   S.total = (int)ids;
+  // PAPER gathers address float4 arrays with 32-bit byte offsets
+  if (params[0].physics == PBF_PHYSICS_PAPER && slot > INT32_MAX / 16)
+    invalid("problem too large for PAPER physics (more than 2^27 slots)");
   const pbf_params &p0 = params[0];
   pbf::StepShape &sh = S.shape;
   sh.slots = (int)slot;

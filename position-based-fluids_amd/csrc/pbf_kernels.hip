@@ -79,6 +79,26 @@ __device__ __forceinline__ float half_hi(unsigned w) {
   return __half2float(__ushort_as_half((unsigned short)(w >> 16)));
 }
 
+// Gathers through a buffer resource: a 32-bit byte offset per lane instead
+// of a 64-bit address, and loads past the n records of the array return 0.
+template <typename T>
+struct Gather {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Gather(const T *p, int n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), (short)0,
+                                            n * (int)sizeof(T), 0x00020000)) {}
+  __device__ __forceinline__ T operator[](int j) const {
+    if constexpr (sizeof(T) == 16) {
+      return __builtin_bit_cast(
+          T, __builtin_amdgcn_raw_buffer_load_b128(r, j * 16, 0, 0));
+    } else {
+      static_assert(sizeof(T) == 8, "16- or 8-byte records");
+      return __builtin_bit_cast(
+          T, __builtin_amdgcn_raw_buffer_load_b64(r, j * 8, 0, 0));
+    }
+  }
+};
+
 // Cell coordinates of a position.  REF: (int)((x - SYSTEM_MIN) /
 // CELL_LENGTH) with IEEE division (update_cells.cl:9-15).  PAPER: the same
 // with a multiply by the reciprocal.
@@ -104,16 +124,13 @@ __device__ __forceinline__ void cell_coords(const Scene &sc, float x, float y,
 // row is cut to the fine x cells within the reach R = (1 + margin) h:
 // with d the distance from xi to the row's y-z rectangle, the fine cells
 // that meet [x - s, x + s], s = sqrt(R^2 - d^2), and no cells if d >= R.
-// Every particle whose bin lies within R of xi is visited.  Returns the
-// sorted slot range [b, e) (b == e: nothing to visit).  A replaying pass
-// recomputes the row starts of the searching pass from the same position, so
-// the arithmetic is pinned: no contraction into FMAs, which the compiler
-// would otherwise choose per call site.
+// Every particle whose bin lies within R of xi is visited (the margin in R
+// covers the rounding of this arithmetic).  Returns the sorted slot range
+// [b, e) (b == e: nothing to visit).
 __device__ __forceinline__ void row_range(const Scene &sc,
                                           const int *__restrict__ cs,
                                           float3 xi, int yy, int zz, int &b,
                                           int &e) {
-#pragma clang fp contract(off)
   const float y0 = sc.bmin[1] + (float)yy * sc.cell_len[1];
   const float z0 = sc.bmin[2] + (float)zz * sc.cell_len[2];
   const float dy = fmaxf(fmaxf(y0 - xi.y, xi.y - (y0 + sc.cell_len[1])), 0.0f);
@@ -122,7 +139,7 @@ __device__ __forceinline__ void row_range(const Scene &sc,
   b = e = 0;
   if (rem > 0.0f) {
     const float u = (xi.x - sc.bmin[0]) * sc.inv_cell[0];
-    const float s = __builtin_sqrtf(rem) * sc.inv_cell[0];
+    const float s = __builtin_amdgcn_sqrtf(rem) * sc.inv_cell[0];
     // clamp in float first: xi may lie far outside the grid
     const float lo = fmaxf(floorf(u - s), 0.0f);
     const float hi = fminf(floorf(u + s), (float)(sc.nc[0] - 1));
@@ -560,8 +577,8 @@ __global__ __launch_bounds__(256) void k_reorder(
 // searches and records N_i in visiting order; the other passes replay it.
 //
 // An entry is 16 bits: stencil row (dz, dy) in the top 4 bits, offset of the
-// slot from the start of that row's slot range in the low 12 (rows and row
-// starts are those of x*_i, recomputed by the replaying pass).  Entries are
+// slot from the row's list base in the low 12 (rows and bases are those of
+// x*_i, kListWindow, recomputed by the replaying pass from the cells alone).  Entries are
 // packed eight to a 16-byte group in registers (a 128-bit shift chain) and
 // each full group is one store; group g of slot i is at nl[g * P + i], so
 // lanes store and load adjacent groups.  A particle with more than kMaxHits
@@ -612,13 +629,23 @@ struct HitRecorder {
 // Stencil walk of a PAPER searching pass: f(j, X[j], row, row_start) for
 // every slot of the 9 row ranges, candidate positions loaded four at a time
 // so that four loads are in flight per wait.
+// Fine x cell at which the list entries of a particle in fine cell cx start
+// counting, in every row.
+__device__ __forceinline__ int list_window(const Scene &sc, int cx) {
+  return min(max(cx - kListWindow, 0), sc.nc[0] - 1);
+}
+
+// Stencil walk of a PAPER searching pass: f(j, X[j], row, list base) for
+// every slot of the 9 row ranges, candidate positions loaded four at a time
+// so that four loads are in flight per wait.
 template <typename F>
 __device__ __forceinline__ void walk_rows(const Scene &sc,
                                           const int *__restrict__ cs,
-                                          const float4 *__restrict__ X,
-                                          float3 xi, F &&f) {
+                                          const Gather<float4> &X, float3 xi,
+                                          F &&f) {
   int cx, cy, cz;
   cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
+  const int win = list_window(sc, cx);
   for (int dz = 0; dz < 3; ++dz) {
     const int zz = cz - 1 + dz;
     if (zz < 0 || zz >= sc.nc[2]) continue;
@@ -627,27 +654,31 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
       if (yy < 0 || yy >= sc.nc[1]) continue;
       int b, e;
       row_range(sc, cs, xi, yy, zz, b, e);
+      if (b >= e) continue;
+      const int lb =
+          cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] + sc.shift;
       const int row = dz * 3 + dy;
       int j = b;
       for (; j + 4 <= e; j += 4) {
         const float4 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2], a3 = X[j + 3];
-        f(j, a0, row, b);
-        f(j + 1, a1, row, b);
-        f(j + 2, a2, row, b);
-        f(j + 3, a3, row, b);
+        f(j, a0, row, lb);
+        f(j + 1, a1, row, lb);
+        f(j + 2, a2, row, lb);
+        f(j + 3, a3, row, lb);
       }
-      for (; j < e; ++j) f(j, X[j], row, b);
+      for (; j < e; ++j) f(j, X[j], row, lb);
     }
   }
 }
 
 // f(j, row, X[j]) for every j of N_i in recorded order, eight entries
-// resolved and their eight loads issued before any is used.  XS = x* (the
-// rows), xs = x*_i; rb is LDS scratch of 9 x blockDim.x ints.
+// resolved and their eight loads issued before any is used; the padding of
+// the last group resolves to j = i, which every pass's 0 < r^2 test (or
+// j != i) drops.  xs = x*_i; rb is LDS scratch of 9 x blockDim.x ints.
 template <typename T, typename F>
 __device__ __forceinline__ void for_neighbours(
-    const Scene &sc, const int *__restrict__ cs, const float4 *__restrict__ XS,
-    const T *__restrict__ X, const uint4 *__restrict__ nl,
+    const Scene &sc, const int *__restrict__ cs, const Gather<float4> &XS,
+    const Gather<T> &X, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, int stride, int i, float3 xs, int *rb,
     F &&f) {
   const int cnt = ncnt[i];
@@ -655,13 +686,15 @@ __device__ __forceinline__ void for_neighbours(
     const int lane = threadIdx.x, bs = blockDim.x;
     int cx, cy, cz;
     cell_coords<kPaper>(sc, xs.x, xs.y, xs.z, cx, cy, cz);
+    const int win = list_window(sc, cx);
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
       const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
-      int start = 0, end = 0;
+      int base = 0;
       if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1])
-        row_range(sc, cs, xs, yy, zz, start, end);
-      rb[r * bs + lane] = start;
+        base = cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] +
+               sc.shift;
+      rb[r * bs + lane] = base;
     }
     for (int g = 0; g * 8 < cnt; ++g) {
       const uint4 w = nl[(size_t)g * stride + i];
@@ -671,15 +704,14 @@ __device__ __forceinline__ void for_neighbours(
       for (int q = 0; q < 8; ++q) {
         const unsigned e = (ws[q >> 1] >> ((q & 1) * 16)) & 0xffffu;
         rows[q] = (int)(e >> 12);
-        js[q] = (g * 8 + q < cnt) ? rb[rows[q] * bs + lane] + (int)(e & 0xfffu)
-                                  : i;
+        const int j = rb[rows[q] * bs + lane] + (int)(e & 0xfffu);
+        js[q] = (g * 8 + q < cnt) ? j : i;
       }
       T xv[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) xv[q] = X[js[q]];
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (g * 8 + q < cnt) f(js[q], rows[q], xv[q]);
+      for (int q = 0; q < 8; ++q) f(js[q], rows[q], xv[q]);
     }
   } else {
     const float h2 = sc.h2;
@@ -841,7 +873,8 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
   HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
-  walk_rows(sc, cs, XS, xi, [&](int j, float4 xj, int row, int b) {
+  const Gather<float4> xsg(XS, stride);
+  walk_rows(sc, cs, xsg, xi, [&](int j, float4 xj, int row, int b) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
@@ -870,7 +903,8 @@ __global__ __launch_bounds__(256) void k_lambda_list(
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
-  for_neighbours(sc, cs, XS, XC, nl, ncnt, stride, i, xs, rb_lds,
+  const Gather<float4> xsg(XS, stride), xcg(XC, stride);
+  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, stride, i, xs, rb_lds,
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
@@ -904,7 +938,9 @@ __global__ __launch_bounds__(256) void k_lambda_list_h(
   const float qz0 = xi.z - (sc.bmin[2] + (cz - 1) * sc.cell_len[2]);
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
-  for_neighbours(sc, cs, XS, XH, nl, ncnt, stride, i, xs, rb_lds,
+  const Gather<float4> xsg(XS, stride);
+  const Gather<uint2> xhg(XH, stride);
+  for_neighbours(sc, cs, xsg, xhg, nl, ncnt, stride, i, xs, rb_lds,
                  [&](int j, int row, uint2 w) {
     const int dz = (row * 11) >> 5;  // row / 3 for row < 9
     const int dy = row - 3 * dz;
@@ -943,7 +979,8 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
   const int n = sc.scorr_n;
-  for_neighbours(sc, cs, XS, B, nl, ncnt, stride, i, xs, rb_lds,
+  const Gather<float4> xsg(XS, stride), bg(B, stride);
+  for_neighbours(sc, cs, xsg, bg, nl, ncnt, stride, i, xs, rb_lds,
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
@@ -952,8 +989,14 @@ __global__ __launch_bounds__(256) void k_delta_paper(
       const float hr = h - r2 * inv;
       const float t = h2 - r2;
       const float q = t * t * t * cq;  // W / W(dq)
-      float qn = 1.0f;
-      for (int e = 0; e < n; ++e) qn *= q;
+      float qn;
+      if (n == 4) {  // the paper's exponent (n is uniform: no divergence)
+        const float q2 = q * q;
+        qn = q2 * q2;
+      } else {
+        qn = 1.0f;
+        for (int e = 0; e < n; ++e) qn *= q;
+      }
       const float coef = li + xj.w - kk * qn;
       const float gs = coef * hr * hr * inv;
       ax += rx * gs;
@@ -994,12 +1037,13 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
   const float4 vi = V[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
   const float h2 = sc.h2;
-  for_neighbours(sc, cs, XS, XC, nl, ncnt, stride, i, xs, rb_lds,
+  const Gather<float4> xsg(XS, stride), xcg(XC, stride), vg(V, stride);
+  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, stride, i, xs, rb_lds,
                  [&](int j, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
-      const float4 vj = V[j];
+      const float4 vj = vg[j];
       const float t = h2 - r2;
       // a neighbour with no density (no pair within h in the last lambda
       // pass) carries no weight
@@ -1035,12 +1079,13 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
   const float4 vi = V[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
   const float h = sc.h, h2 = sc.h2;
-  for_neighbours(sc, cs, XS, XC, nl, ncnt, stride, i, xs, rb_lds,
+  const Gather<float4> xsg(XS, stride), xcg(XC, stride), vg(V, stride);
+  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, stride, i, xs, rb_lds,
                  [&](int j, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
-      const float4 vj = V[j];
+      const float4 vj = vg[j];
       const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
       const float t = h2 - r2;
       const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
@@ -1088,7 +1133,8 @@ __global__ __launch_bounds__(256) void k_vort_final(
   const float4 wi = OM[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   const float h = sc.h, h2 = sc.h2;
-  for_neighbours(sc, cs, XS, B, nl, ncnt, stride, i, xs, rb_lds,
+  const Gather<float4> xsg(XS, stride), bg(B, stride);
+  for_neighbours(sc, cs, xsg, bg, nl, ncnt, stride, i, xs, rb_lds,
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
@@ -1330,7 +1376,7 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
   auto fp16 = [&](int it) { return it >= 1 && it <= sh.fp16_iterations; };
   for (int it = 0; it < sh.iterations; ++it) {
     if (it == 0) {
-      PBF_LAUNCH(L, kClsLambda, k_lambda_paper, grid, bs, d.scenes,
+      PBF_LAUNCH(L, kClsSearch, k_lambda_paper, grid, bs, d.scenes,
                  d.block_scene, XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt,
                  sh.slots);
     } else if (fp16(it)) {
@@ -1422,8 +1468,8 @@ void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
 
 const char *kernel_class_name(int cls) {
   static const char *names[kNumClasses] = {
-      "predict_count", "scan", "scatter", "reorder", "lambda",
-      "delta",         "xsph", "vorticity"};
+      "predict_count", "scan", "scatter",   "reorder", "lambda",
+      "delta",         "xsph", "vorticity", "search"};
   return (cls >= 0 && cls < kNumClasses) ? names[cls] : "?";
 }
 

@@ -23,6 +23,7 @@ enum KernelClass {
   kClsDelta,
   kClsXsph,
   kClsVort,
+  kClsSearch,  // PAPER: the lambda pass that searches and records N_i
   kNumClasses
 };
 

@@ -21,6 +21,14 @@ constexpr int kXSub = 4;
 // row cut (the search runs at the positions the bins were built from).
 constexpr float kReachMargin = 1e-3f;
 
+// Neighbour-list entries are offsets from the first slot of fine x cell
+// max(fx - kListWindow, 0) of their row, fx the particle's own fine x cell:
+// the reach spans at most kXSub (1 + kReachMargin) fine cells, so every
+// visited slot lies at or after that cell, and a replaying pass finds the
+// base from fx alone.
+constexpr int kListWindow = kXSub + 1;
+static_assert(kReachMargin * kXSub < 1.0f, "reach must stay within the window");
+
 // Capacity of a particle's neighbour list (pairs within h at x*).
 constexpr int kMaxHits = 96;
 static_assert(kMaxHits % 8 == 0, "hit lists are stored in groups of 8");

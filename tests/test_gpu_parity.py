@@ -424,7 +424,9 @@ def test_profile_and_timing_hooks():
     say("profile hooks")
     with pbf.Simulation(sc.params[0], sc.particles[0]) as s:
         prof = s.profile(2)
-        assert prof["lambda"][1] == 8 and prof["delta"][1] == 8
+        # per step: 1 searching lambda pass, 3 replaying ones, 4 delta passes
+        assert prof["search"][1] == 2 and prof["lambda"][1] == 6
+        assert prof["delta"][1] == 8
         assert all(ms >= 0 for ms, _ in prof.values())
         assert s.time_steps(3) > 0
         assert s.config()[0] in (64, 128, 256)
