@@ -77,11 +77,26 @@ def step_bytes(n: int, cells: int, iterations: int, physics: int,
 
 
 def load_traffic(workload: str):
+    """Measured HBM bytes per launch of each kernel class (FETCH_SIZE +
+    WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), written
+    by --rocprof-summary ... --traffic-out."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f)
+
+
+# rocprof kernel name -> kernel class of pbf_profile_steps
+def kernel_class(name: str):
+    for prefix, cls in (("k_lambda_paper", "search"), ("k_lambda_list", "lambda"),
+                        ("k_lambda_ref", "lambda"), ("k_delta", "delta"),
+                        ("k_xsph", "xsph"), ("k_vort", "vorticity"),
+                        ("k_predict_count", "predict_count"), ("k_scan", "scan"),
+                        ("k_scatter", "scatter"), ("k_reorder", "reorder")):
+        if name.startswith(prefix):
+            return cls
+    return None
 
 
 def cpu_baseline(params, parts, steps: int, threads: int):
@@ -101,9 +116,11 @@ def _short(name: str) -> str:
     return name.split("(")[0]
 
 
-def rocprof_summary(root: str, particles: int, copy_mib: int) -> str:
+def rocprof_summary(root: str, particles: int, copy_mib: int,
+                    traffic_out: str = None) -> str:
     """Markdown summary of the rocprofv3 databases written by the profiling
-    pass (sub-directories kt, fetch, write, valu, cfetch, cwrite)."""
+    pass (sub-directories kt, fetch, write, valu, cfetch, cwrite); with
+    traffic_out, also writes the per-class HBM bytes per launch as JSON."""
     import glob
     import sqlite3
 
@@ -159,6 +176,19 @@ def rocprof_summary(root: str, particles: int, copy_mib: int) -> str:
                    f"{per(fetch.get(k), fscale)} | {per(write.get(k), wscale)} | "
                    f"{'-' if vi is None else f'{vi * 64 / particles:.0f}'} | "
                    f"{'-' if vm is None else f'{vm * 64 / particles:.0f}'} |")
+    if traffic_out and fscale and wscale:
+        per_cls = {}
+        for k in times:
+            cls = kernel_class(k)
+            if cls and k in fetch and k in write:
+                b = (fetch[k] * fscale + write[k] * wscale) * 1024.0
+                calls = times[k][0]
+                n, tot = per_cls.get(cls, (0, 0.0))
+                per_cls[cls] = (n + calls, tot + b * calls)
+        with open(traffic_out, "w") as f:
+            json.dump({"particles": particles,
+                       "bytes_per_launch": {c: tot / n for c, (n, tot) in per_cls.items()}},
+                      f, indent=1)
     return "\n".join(out)
 
 
@@ -300,12 +330,15 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain float4 copy of MIB MiB (counter "
                          "calibration) and print its time")
+    ap.add_argument("--traffic-out", metavar="JSON",
+                    help="with --rocprof-summary: write per-class HBM bytes "
+                         "per launch")
     ap.add_argument("--rocprof-summary", metavar="DIR",
                     help="summarise the rocprofv3 databases under DIR and exit")
     args = ap.parse_args()
     if args.rocprof_summary:
         n = scenarios.config(args.config).n if args.config != 3 else 0
-        print(rocprof_summary(args.rocprof_summary, n, 512))
+        print(rocprof_summary(args.rocprof_summary, n, 512, args.traffic_out))
         return
 
     if args.gpus != 1:
@@ -390,7 +423,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (traffic or {}).get("bytes_per_launch"),
+            "traffic": (traffic or {}).get("bytes_per_launch", {}).get(dom),
             "avg_launch_ms": avg_ms,
             "algorithmic_bytes_per_launch": dom_bytes,
         },

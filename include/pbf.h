@@ -219,12 +219,14 @@ int pbf_time_steps(pbf_sim *sim, uint32_t nsteps, double *ms);
  * the per-kernel FETCH_SIZE / WRITE_SIZE readings. */
 int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms);
 
-/* Launch configuration of this handle: cfg[0] = gather work-group size
- * (chosen by timing 64, 128 and 256 on the first step when block_size is
- * 0 and the handle has at least 2^18 slots; cached per shape, and in the
- * file named by the PBF_TUNE_CACHE environment variable), cfg[1] =
- * particles per work-item (1), cfg[2] = sort passes (1: one counting pass),
- * cfg[3] = FP16 lambda passes per step. */
+/* Launch configuration of this handle: cfg[0] = work-group size of the
+ * searching lambda pass (PAPER) or of the lambda pass (REF).  With
+ * block_size 0 and at least 2^18 slots, every gather kernel class gets its
+ * own size, chosen by timing 64, 128 and 256 on the first step (cached per
+ * shape, and in the file named by the PBF_TUNE_CACHE environment variable);
+ * otherwise all use block_size (default 128).  cfg[1] = particles per
+ * work-item (1), cfg[2] = sort passes (1: one counting pass), cfg[3] = FP16
+ * lambda passes per step. */
 int pbf_get_config(const pbf_sim *sim, int32_t cfg[4]);
 
 #ifdef __cplusplus

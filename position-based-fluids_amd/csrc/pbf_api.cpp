@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -320,7 +321,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   pbf::StepShape &sh = S.shape;
   sh.slots = (int)slot;
   sh.cells = (int)cells;
-  sh.scan_blocks = (int)((cells + pbf::kScanTile - 1) / pbf::kScanTile);
+  sh.scan_blocks = (int)((cells + pbf::kScanTile) / pbf::kScanTile);
   sh.physics = p0.physics == PBF_PHYSICS_REF ? pbf::kRef : pbf::kPaper;
   sh.iterations = p0.iterations;
   // iteration 0 searches in FP32; iterations 1 .. I - F - 1 are FP16
@@ -336,10 +337,11 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   for (uint32_t s = 0; s < nscenes; ++s)
     vort = vort || params[s].vorticity_eps != 0.0f;
   sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
-  sh.block = p0.block_size ? p0.block_size : 128;
+  for (int c = 0; c < pbf::kNumClasses; ++c)
+    sh.block[c] = p0.block_size ? p0.block_size : 128;
   S.tune_pending = p0.block_size == 0;
 
-  const size_t P = (size_t)sh.slots, M = (size_t)sh.cells;
+  const size_t P = (size_t)sh.slots;
   pbf::DevBuffers &d = S.d;
   d.pos = S.alloc<float4>(P);
   d.vel = S.alloc<float4>(P);
@@ -347,8 +349,9 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   d.pred_u = S.alloc<float4>(P);
   d.key = S.alloc<int>(P);
   d.rank = S.alloc<int>(P);
-  d.count = S.alloc<int>(M);
-  d.cell_start = S.alloc<int>(M + 1);
+  const size_t Mpad = (size_t)sh.scan_blocks * pbf::kScanTile;  // >= M + 1
+  d.count = S.alloc<int>(Mpad);
+  d.cell_start = S.alloc<int>(Mpad);
   d.partial = S.alloc<int>((size_t)sh.scan_blocks + 1);
   d.perm = S.alloc<int>(P);
   d.ppid = S.alloc<int>(P);
@@ -400,7 +403,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
                     hipMemcpyHostToDevice));
   PBF_HIP(hipMemcpy(d.pid, hpid.data(), P * sizeof(int),
                     hipMemcpyHostToDevice));
-  PBF_HIP(hipMemset(d.count, 0, M * sizeof(int)));
+  PBF_HIP(hipMemset(d.count, 0, Mpad * sizeof(int)));
   PBF_HIP(hipMemset(d.err, 0, sizeof(int)));
   PBF_HIP(hipMemcpy(d.block_scene, S.block_scene.data(),
                     S.block_scene.size() * sizeof(int),
@@ -410,16 +413,19 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
 }
 
 // ---- autotuner ---------------------------------------------------------
-// Chooses the work-group size of the gather kernels (64, 128 or 256) by
-// timing two steps of each on the handle's own state, saved before and
-// restored after.  Results are kept per problem shape for the process and,
-// when PBF_TUNE_CACHE names a file, in that file ("key block" lines).
-// Handles below kTuneMinSlots slots keep the default: there the launch
-// overhead, not the kernels, dominates.
+// Chooses the work-group size (64, 128 or 256) of each gather kernel class
+// by timing each class over two steps at every size, on the handle's own
+// state (saved before and restored after); the classes are independent
+// launches, so the fastest size of each is taken.  Results are kept per
+// problem shape for the process and, when PBF_TUNE_CACHE names a file, in
+// that file ("key size size ..." lines, one size per class).  Handles below
+// kTuneMinSlots slots keep the default: there the launch overhead, not the
+// kernels, dominates.
 constexpr int kTuneMinSlots = 1 << 18;
+using Blocks = std::array<int, pbf::kNumClasses>;
 
 std::mutex g_tune_mu;
-std::map<std::string, int> g_tune;
+std::map<std::string, Blocks> g_tune;
 
 std::string tune_key(const pbf_sim &S) {
   hipDeviceProp_t prop;
@@ -429,33 +435,48 @@ std::string tune_key(const pbf_sim &S) {
   std::ostringstream k;
   k << arch << ":" << S.shape.slots << ":" << S.shape.cells << ":"
     << S.shape.physics << ":" << S.shape.iterations << ":"
-    << S.shape.vorticity << ":" << S.scenes.size();
+    << S.shape.fp16_iterations << ":" << S.shape.vorticity << ":"
+    << S.scenes.size() << ":c" << pbf::kNumClasses;
   return k.str();
 }
 
-int tune_lookup(const std::string &key) {
+bool valid_block(int b) { return b == 64 || b == 128 || b == 256; }
+
+bool tune_lookup(const std::string &key, Blocks &out) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tune.find(key);
-  if (it != g_tune.end()) return it->second;
+  if (it != g_tune.end()) {
+    out = it->second;
+    return true;
+  }
   if (const char *path = std::getenv("PBF_TUNE_CACHE")) {
     std::ifstream f(path);
-    std::string k;
-    int b;
-    while (f >> k >> b)
-      if (k == key && (b == 64 || b == 128 || b == 256)) {
+    std::string line;
+    while (std::getline(f, line)) {
+      std::istringstream ls(line);
+      std::string k;
+      Blocks b;
+      if (!(ls >> k) || k != key) continue;
+      bool ok = true;
+      for (int &v : b) ok = ok && (ls >> v) && valid_block(v);
+      if (ok) {
         g_tune[key] = b;
-        return b;
+        out = b;
+        return true;
       }
+    }
   }
-  return 0;
+  return false;
 }
 
-void tune_store(const std::string &key, int block) {
+void tune_store(const std::string &key, const Blocks &b) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
-  g_tune[key] = block;
+  g_tune[key] = b;
   if (const char *path = std::getenv("PBF_TUNE_CACHE")) {
     std::ofstream f(path, std::ios::app);
-    f << key << " " << block << "\n";
+    f << key;
+    for (int v : b) f << " " << v;
+    f << "\n";
   }
 }
 
@@ -464,8 +485,9 @@ void autotune(pbf_sim &S) {
   S.tune_pending = false;
   if (S.shape.slots < kTuneMinSlots) return;
   const std::string key = tune_key(S);
-  if (const int b = tune_lookup(key)) {
-    S.shape.block = b;
+  Blocks cached;
+  if (tune_lookup(key, cached)) {
+    std::copy(cached.begin(), cached.end(), S.shape.block);
     return;
   }
   const size_t P = (size_t)S.shape.slots;
@@ -479,50 +501,49 @@ void autotune(pbf_sim &S) {
   auto copy = [&](void *dst, const void *src, size_t bytes) {
     PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S.stream));
   };
-  hipEvent_t e0 = nullptr, e1 = nullptr;
   try {
     PBF_HIP(hipMalloc(&spos, P * sizeof(float4)));
     PBF_HIP(hipMalloc(&svel, P * sizeof(float4)));
     PBF_HIP(hipMalloc(&spid, P * sizeof(int)));
-    PBF_HIP(hipEventCreate(&e0));
-    PBF_HIP(hipEventCreate(&e1));
     copy(spos, S.d.pos, P * sizeof(float4));
     copy(svel, S.d.vel, P * sizeof(float4));
     copy(spid, S.d.pid, P * sizeof(int));
-    float best = 1e30f;
-    int best_b = S.shape.block;
+    Blocks best_b;
+    std::array<double, pbf::kNumClasses> best_ms;
+    best_b.fill(S.shape.block[0]);
+    best_ms.fill(1e30);
     for (int b : {64, 128, 256}) {
-      S.shape.block = b;
+      for (int c = 0; c < pbf::kNumClasses; ++c) S.shape.block[c] = b;
       copy(S.d.pos, spos, P * sizeof(float4));
       copy(S.d.vel, svel, P * sizeof(float4));
       copy(S.d.pid, spid, P * sizeof(int));
       pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);  // warm-up
-      PBF_HIP(hipEventRecord(e0, S.stream));
+      EventTimer timer;
       for (int t = 0; t < 2; ++t)
-        pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);
-      PBF_HIP(hipEventRecord(e1, S.stream));
-      PBF_HIP(hipEventSynchronize(e1));
-      float ms = 0.0f;
-      PBF_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        best_b = b;
+        pbf::enqueue_step(S.d, S.shape, S.stream, &timer);
+      PBF_HIP(hipStreamSynchronize(S.stream));
+      std::array<double, pbf::kNumClasses> ms{};
+      for (const auto &r : timer.recs) {
+        float t = 0.0f;
+        PBF_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        ms[r.cls] += t;
       }
+      for (int c = 0; c < pbf::kNumClasses; ++c)
+        if (ms[c] < best_ms[c]) {
+          best_ms[c] = ms[c];
+          best_b[c] = b;
+        }
     }
     copy(S.d.pos, spos, P * sizeof(float4));
     copy(S.d.vel, svel, P * sizeof(float4));
     copy(S.d.pid, spid, P * sizeof(int));
     PBF_HIP(hipStreamSynchronize(S.stream));
-    S.shape.block = best_b;
+    std::copy(best_b.begin(), best_b.end(), S.shape.block);
     tune_store(key, best_b);
   } catch (...) {
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
     release();
     throw;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   release();
 }
 
@@ -900,7 +921,9 @@ int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms) {
 int pbf_get_config(const pbf_sim *S, int32_t cfg[4]) {
   return guard([&] {
     need(S && cfg, "NULL argument");
-    cfg[0] = S->shape.block;
+    cfg[0] = S->shape.block[S->shape.physics == pbf::kPaper
+                                ? pbf::kClsSearch
+                                : pbf::kClsLambda];
     cfg[1] = 1;
     cfg[2] = 1;
     cfg[3] = S->shape.fp16_iterations;

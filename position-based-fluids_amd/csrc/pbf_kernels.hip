@@ -87,15 +87,18 @@ struct Gather {
   __device__ __forceinline__ Gather(const T *p, int n)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), (short)0,
                                             n * (int)sizeof(T), 0x00020000)) {}
-  __device__ __forceinline__ T operator[](int j) const {
+  __device__ __forceinline__ T at_bytes(int off) const {
     if constexpr (sizeof(T) == 16) {
       return __builtin_bit_cast(
-          T, __builtin_amdgcn_raw_buffer_load_b128(r, j * 16, 0, 0));
+          T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
     } else {
       static_assert(sizeof(T) == 8, "16- or 8-byte records");
       return __builtin_bit_cast(
-          T, __builtin_amdgcn_raw_buffer_load_b64(r, j * 8, 0, 0));
+          T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
+  }
+  __device__ __forceinline__ T operator[](int j) const {
+    return at_bytes(j * (int)sizeof(T));
   }
 };
 
@@ -454,26 +457,17 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
 }
 
 __global__ __launch_bounds__(256) void k_scan_reduce(
-    const int *__restrict__ count, int m, int *__restrict__ partial) {
+    const int4 *__restrict__ count, int *__restrict__ partial) {
   __shared__ int lds[4];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanIpt;
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanIpt; ++k) {
-    const int idx = base + k;
-    s += (idx < m) ? count[idx] : 0;
-  }
+  const int4 c = count[blockIdx.x * 256 + threadIdx.x];
   int tot;
-  block_excl_scan256(s, lds, tot);
+  block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot);
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// One block: exclusive scan of the block sums in place; writes the grand
-// total to cell_start[m].
+// One block: exclusive scan of the nb block sums in place.
 __global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ partial,
-                                                  int nb,
-                                                  int *__restrict__ cell_start,
-                                                  int m) {
+                                                  int nb) {
   __shared__ int lds[4];
   int carry = 0;
   for (int base = 0; base < nb; base += 256 * kScanIpt) {
@@ -495,33 +489,22 @@ __global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ partial,
     }
     carry += tot;
   }
-  if (threadIdx.x == 0) cell_start[m] = carry;
 }
 
+// cell_start = exclusive scan of the counts; the counts are zeroed for the
+// next step.
 __global__ __launch_bounds__(256) void k_scan_down(
-    int *__restrict__ count, int m, const int *__restrict__ partial,
-    int *__restrict__ cell_start) {
+    int4 *__restrict__ count, const int *__restrict__ partial,
+    int4 *__restrict__ cell_start) {
   __shared__ int lds[4];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanIpt;
-  int v[kScanIpt];
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanIpt; ++k) {
-    const int idx = base + k;
-    v[k] = (idx < m) ? count[idx] : 0;
-    s += v[k];
-  }
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int4 c = count[t];
   int tot;
-  int run = block_excl_scan256(s, lds, tot) + partial[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kScanIpt; ++k) {
-    const int idx = base + k;
-    if (idx < m) {
-      cell_start[idx] = run;
-      count[idx] = 0;
-    }
-    run += v[k];
-  }
+  const int run = block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot) +
+                  partial[blockIdx.x];
+  cell_start[t] = make_int4(run, run + c.x, run + c.x + c.y,
+                            run + c.x + c.y + c.z);
+  count[t] = make_int4(0, 0, 0, 0);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(
@@ -610,17 +593,23 @@ struct HitRecorder {
     }
     ++c;
   }
-  // store the last partial group (aligned to its first entry) and the count
+  // Store the last partial group and the count.  The group is padded with
+  // entries naming slot i itself (stencil row 4 is the particle's own row;
+  // self_off = i - that row's list base, < 0 when it has none), which every
+  // replaying pass drops (r = 0), so a replay decodes all eight entries of
+  // a group without a count test.
   __device__ __forceinline__ void flush(uint4 *__restrict__ nl,
                                        int *__restrict__ ncnt, int stride,
-                                       int i) {
+                                       int i, int self_off) {
+    if ((c & 7) && (self_off < 0 || self_off >= 4096)) over = true;
     if (over) {
       ncnt[i] = kMaxHits + 1;
       return;
     }
     ncnt[i] = c;
     if (c & 7) {
-      for (int k = c & 7; k < 8; ++k) push(0u);
+      const unsigned pad = (4u << 12) | (unsigned)self_off;
+      for (int k = c & 7; k < 8; ++k) push(pad);
       nl[(size_t)(c >> 3) * stride + i] = make_uint4(w0, w1, w2, w3);
     }
   }
@@ -673,7 +662,7 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
 
 // f(j, row, X[j]) for every j of N_i in recorded order, eight entries
 // resolved and their eight loads issued before any is used; the padding of
-// the last group resolves to j = i, which every pass's 0 < r^2 test (or
+// the last group names i itself, which every pass's 0 < r^2 test (or
 // j != i) drops.  xs = x*_i; rb is LDS scratch of 9 x blockDim.x ints.
 template <typename T, typename F>
 __device__ __forceinline__ void for_neighbours(
@@ -681,9 +670,13 @@ __device__ __forceinline__ void for_neighbours(
     const Gather<T> &X, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, int stride, int i, float3 xs, int *rb,
     F &&f) {
+  constexpr int kShift = sizeof(T) == 16 ? 4 : 3;  // bytes per record, log2
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
-    const int lane = threadIdx.x, bs = blockDim.x;
+    // rb[row * bs + lane] = byte offset of the row's list base in X (lanes
+    // of one row in consecutive banks)
+    const int bs = blockDim.x;
+    int *my = rb + threadIdx.x;
     int cx, cy, cz;
     cell_coords<kPaper>(sc, xs.x, xs.y, xs.z, cx, cy, cz);
     const int win = list_window(sc, cx);
@@ -694,24 +687,23 @@ __device__ __forceinline__ void for_neighbours(
       if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1])
         base = cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] +
                sc.shift;
-      rb[r * bs + lane] = base;
+      my[r * bs] = base << kShift;
     }
     for (int g = 0; g * 8 < cnt; ++g) {
       const uint4 w = nl[(size_t)g * stride + i];
       const unsigned ws[4] = {w.x, w.y, w.z, w.w};
-      int js[8], rows[8];
+      int off[8], rows[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const unsigned e = (ws[q >> 1] >> ((q & 1) * 16)) & 0xffffu;
         rows[q] = (int)(e >> 12);
-        const int j = rb[rows[q] * bs + lane] + (int)(e & 0xfffu);
-        js[q] = (g * 8 + q < cnt) ? j : i;
+        off[q] = my[rows[q] * bs] + ((int)(e & 0xfffu) << kShift);
       }
       T xv[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) xv[q] = X[js[q]];
+      for (int q = 0; q < 8; ++q) xv[q] = X.at_bytes(off[q]);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) f(js[q], rows[q], xv[q]);
+      for (int q = 0; q < 8; ++q) f(off[q] >> kShift, rows[q], xv[q]);
     }
   } else {
     const float h2 = sc.h2;
@@ -882,7 +874,15 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
       ls.add(h, h2, rx, ry, rz, r2);
     }
   });
-  rec.flush(nl, ncnt, stride, i);
+  // list base of the particle's own row (stencil row 4), for the padding
+  int cx, cy, cz;
+  cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
+  int self_off = -1;
+  if (cy >= 0 && cy < sc.nc[1] && cz >= 0 && cz < sc.nc[2])
+    self_off = i - (cs[sc.cell_base + (cz * sc.nc[1] + cy) * sc.nc[0] +
+                       list_window(sc, cx)] +
+                    sc.shift);
+  rec.flush(nl, ncnt, stride, i, self_off);
   ls.store(sc, i, xi, B, rho);
 }
 
@@ -1318,6 +1318,13 @@ struct Launcher {
   } while (0)
 #define PBF_LAUNCH(L, cls, kern, grid, block, ...) \
   PBF_LAUNCH_SMEM(L, cls, kern, grid, block, 0, __VA_ARGS__)
+// A per-particle gather kernel at its class's work-group size, with
+// lds_ints ints of LDS per work-item (the replay row bases).
+#define PBF_GATHER(L, cls, kern, lds_ints, ...)                          \
+  PBF_LAUNCH_SMEM(L, cls, kern, (L).sh.slots / (L).sh.block[cls],        \
+                  (L).sh.block[cls],                                     \
+                  (size_t)(lds_ints) * (L).sh.block[cls] * sizeof(int),  \
+                  __VA_ARGS__)
 
 template <int PH>
 static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
@@ -1327,12 +1334,12 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
   PBF_LAUNCH(L, kClsPredict, k_predict_count<PH>, grid, bs, d.scenes,
              d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank, d.count,
              predict);
-  PBF_LAUNCH(L, kClsScan, k_scan_reduce, sh.scan_blocks, 256, d.count, sh.cells,
-             d.partial);
-  PBF_LAUNCH(L, kClsScan, k_scan_top, 1, 256, d.partial, sh.scan_blocks,
-             d.cell_start, sh.cells);
-  PBF_LAUNCH(L, kClsScan, k_scan_down, sh.scan_blocks, 256, d.count, sh.cells,
-             d.partial, d.cell_start);
+  PBF_LAUNCH(L, kClsScan, k_scan_reduce, sh.scan_blocks, 256,
+             reinterpret_cast<const int4 *>(d.count), d.partial);
+  PBF_LAUNCH(L, kClsScan, k_scan_top, 1, 256, d.partial, sh.scan_blocks);
+  PBF_LAUNCH(L, kClsScan, k_scan_down, sh.scan_blocks, 256,
+             reinterpret_cast<int4 *>(d.count), d.partial,
+             reinterpret_cast<int4 *>(d.cell_start));
   PBF_LAUNCH(L, kClsScatter, k_scatter, grid, bs, d.scenes, d.block_scene,
              d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
@@ -1342,81 +1349,67 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
 
 static void step_ref(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
-  const int bs = sh.block;
-  const int grid = sh.slots / bs;
   sort_stage<kRef>(d, L, 1);
   float4 *cur = d.xa, *nxt = d.xb;
   for (int it = 0; it < sh.iterations; ++it) {
-    PBF_LAUNCH(L, kClsLambda, k_lambda_ref, grid, bs, d.scenes, d.block_scene,
-               cur, d.cell_start, d.lam, d.rho);
+    PBF_GATHER(L, kClsLambda, k_lambda_ref, 0, d.scenes, d.block_scene, cur,
+               d.cell_start, d.lam, d.rho);
     if (it == sh.iterations - 1) {
-      PBF_LAUNCH(L, kClsDelta, k_delta_ref<true>, grid, bs, d.scenes,
-                 d.block_scene, cur, d.cell_start, d.lam, d.rho, d.xold, nxt,
-                 d.v);
+      PBF_GATHER(L, kClsDelta, k_delta_ref<true>, 0, d.scenes, d.block_scene,
+                 cur, d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
     } else {
-      PBF_LAUNCH(L, kClsDelta, k_delta_ref<false>, grid, bs, d.scenes,
-                 d.block_scene, cur, d.cell_start, d.lam, d.rho, d.xold, nxt,
-                 d.v);
+      PBF_GATHER(L, kClsDelta, k_delta_ref<false>, 0, d.scenes, d.block_scene,
+                 cur, d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
     }
     std::swap(cur, nxt);
   }
-  PBF_LAUNCH(L, kClsXsph, k_xsph_final_ref, grid, bs, d.scenes, d.block_scene,
-             cur, d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err);
+  PBF_GATHER(L, kClsXsph, k_xsph_final_ref, 0, d.scenes, d.block_scene, cur,
+             d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err);
 }
 
 static void step_paper(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
-  const int bs = sh.block;
-  const int grid = sh.slots / bs;
   sort_stage<kPaper>(d, L, 1);
-  const size_t rb_smem = (size_t)9 * bs * sizeof(int);  // replay row starts
   const float4 *XS = d.xa;
+  const int P = sh.slots;
   // iteration 0's lambda pass is the FP32 search; mixed precision runs the
   // lambda passes of iterations 1 .. fp16_iterations over FP16 positions
   auto fp16 = [&](int it) { return it >= 1 && it <= sh.fp16_iterations; };
   for (int it = 0; it < sh.iterations; ++it) {
     if (it == 0) {
-      PBF_LAUNCH(L, kClsSearch, k_lambda_paper, grid, bs, d.scenes,
-                 d.block_scene, XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt,
-                 sh.slots);
+      PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene,
+                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, P);
     } else if (fp16(it)) {
-      PBF_LAUNCH_SMEM(L, kClsLambda, k_lambda_list_h, grid, bs, rb_smem,
-                      d.scenes, d.block_scene, XS, d.xc, d.xh, d.cell_start,
-                      d.nl, d.ncnt, sh.slots, d.xb, d.rho);
+      PBF_GATHER(L, kClsLambda, k_lambda_list_h, 9, d.scenes, d.block_scene,
+                 XS, d.xc, d.xh, d.cell_start, d.nl, d.ncnt, P, d.xb, d.rho);
     } else {
-      PBF_LAUNCH_SMEM(L, kClsLambda, k_lambda_list, grid, bs, rb_smem,
-                      d.scenes, d.block_scene, XS, d.xc, d.cell_start, d.nl,
-                      d.ncnt, sh.slots, d.xb, d.rho);
+      PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
+                 XS, d.xc, d.cell_start, d.nl, d.ncnt, P, d.xb, d.rho);
     }
     if (it == sh.iterations - 1) {
-      PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<true, false>), grid, bs,
-                      rb_smem, d.scenes, d.block_scene, XS, d.xb,
-                      d.cell_start, d.rho, d.xold, d.nl, d.ncnt, sh.slots,
-                      d.kc, d.xc, d.v, d.xh);
+      PBF_GATHER(L, kClsDelta, (k_delta_paper<true, false>), 9, d.scenes,
+                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
+                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
     } else if (fp16(it + 1)) {
-      PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<false, true>), grid, bs,
-                      rb_smem, d.scenes, d.block_scene, XS, d.xb,
-                      d.cell_start, d.rho, d.xold, d.nl, d.ncnt, sh.slots,
-                      d.kc, d.xc, d.v, d.xh);
+      PBF_GATHER(L, kClsDelta, (k_delta_paper<false, true>), 9, d.scenes,
+                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
+                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
     } else {
-      PBF_LAUNCH_SMEM(L, kClsDelta, (k_delta_paper<false, false>), grid, bs,
-                      rb_smem, d.scenes, d.block_scene, XS, d.xb,
-                      d.cell_start, d.rho, d.xold, d.nl, d.ncnt, sh.slots,
-                      d.kc, d.xc, d.v, d.xh);
+      PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
+                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
+                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
     }
   }
   if (sh.vorticity) {
-    PBF_LAUNCH_SMEM(L, kClsXsph, k_xsph_omega, grid, bs, rb_smem, d.scenes,
-                    d.block_scene, XS, d.xc, d.cell_start, d.v, d.nl, d.ncnt,
-                    sh.slots, d.vx, d.om, d.xb);
-    PBF_LAUNCH_SMEM(L, kClsVort, k_vort_final, grid, bs, rb_smem, d.scenes,
-                    d.block_scene, XS, d.xb, d.cell_start, d.vx, d.om, d.nl,
-                    d.ncnt, sh.slots, d.pid_s, d.pos, d.vel, d.pid, d.err);
+    PBF_GATHER(L, kClsXsph, k_xsph_omega, 9, d.scenes, d.block_scene, XS,
+               d.xc, d.cell_start, d.v, d.nl, d.ncnt, P, d.vx, d.om, d.xb);
+    PBF_GATHER(L, kClsVort, k_vort_final, 9, d.scenes, d.block_scene, XS,
+               d.xb, d.cell_start, d.vx, d.om, d.nl, d.ncnt, P, d.pid_s,
+               d.pos, d.vel, d.pid, d.err);
   } else {
-    PBF_LAUNCH_SMEM(L, kClsXsph, k_xsph_final_paper, grid, bs, rb_smem,
-                    d.scenes, d.block_scene, XS, d.xc, d.cell_start, d.v,
-                    d.nl, d.ncnt, sh.slots, d.pid_s, d.pos, d.vel, d.pid,
-                    d.err);
+    PBF_GATHER(L, kClsXsph, k_xsph_final_paper, 9, d.scenes, d.block_scene,
+               XS, d.xc, d.cell_start, d.v, d.nl, d.ncnt, P, d.pid_s, d.pos,
+               d.vel, d.pid, d.err);
   }
 }
 
@@ -1431,7 +1424,7 @@ void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
 
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
   Launcher L{sh, st, nullptr};
-  const int bs = sh.block, grid = sh.slots / bs;
+  const int bs = 256, grid = sh.slots / bs;
   if (sh.physics == kRef) {
     sort_stage<kRef>(d, L, 1);
     k_count_nbrs<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,

@@ -10,7 +10,11 @@ constexpr int kRef = 0;
 constexpr int kPaper = 1;
 
 // scan tiling: 256 threads x 16 counts per block
-constexpr int kScanIpt = 16;
+// Cell-count scan: 256 work-items x 4 counts (one int4) per work-group.
+// The count and cell_start arrays are allocated to a whole number of tiles
+// of at least M + 1 entries (the padding counts stay 0), so the scan needs
+// no bounds tests and writes cell_start[M] = total by itself.
+constexpr int kScanIpt = 4;
 constexpr int kScanTile = 256 * kScanIpt;
 
 // kernel classes reported by the profiler
@@ -31,12 +35,12 @@ enum KernelClass {
 struct StepShape {
   int slots;        // padded particle slots P (multiple of kSlotAlign)
   int cells;        // M, total cells over scenes
-  int scan_blocks;  // ceil(M / kScanTile)
+  int scan_blocks;  // ceil((M + 1) / kScanTile)
   int physics;      // kRef / kPaper (uniform over scenes)
   int iterations;
   int fp16_iterations;  // leading iterations gathering FP16 positions
   int vorticity;    // PAPER: vorticity confinement pass enabled
-  int block;        // gather work-group size (64, 128 or 256)
+  int block[kNumClasses];  // work-group size (64, 128 or 256) per class
 };
 
 // Per-kernel-class event timing (pbf_profile_steps).
