@@ -200,9 +200,10 @@ uint32_t pbf_num_cells(const pbf_sim *sim);
  * The density is evaluated at the current positions. */
 int pbf_diagnostics(pbf_sim *sim, double out[8]);
 
-/* Run nsteps steps bracketing every kernel launch with HIP events on the
- * handle's stream; writes per-kernel-class totals (milliseconds, summed
- * over the nsteps steps) into ms[k] and launch counts into launches[k] for
+/* Run nsteps steps, timing every kernel launch with start/stop events
+ * attached to its dispatch (the kernel's own duration, launch gaps
+ * excluded); writes per-kernel-class totals (milliseconds, summed over the
+ * nsteps steps) into ms[k] and launch counts into launches[k] for
  * k < *nclass (in: capacity, out: number of classes), names[k] -> static
  * strings.  The state advances as with pbf_step. */
 int pbf_profile_steps(pbf_sim *sim, uint32_t nsteps, double *ms,
@@ -212,6 +213,14 @@ int pbf_profile_steps(pbf_sim *sim, uint32_t nsteps, double *ms,
 /* Time nsteps graph launches with HIP events on the handle's stream
  * (after the caller's warm-up); returns elapsed milliseconds in *ms. */
 int pbf_time_steps(pbf_sim *sim, uint32_t nsteps, double *ms);
+
+/* Average duration in *ms of one launch of the kernel class named cls
+ * ("search", "lambda" or "delta"; REF: "lambda" or "delta"): after one
+ * step, reps launches of that class's kernel are captured back to back into
+ * a graph and timed with events around one graph launch.  The kernels
+ * recompute step temporaries only; the simulation state (pbf_dump) is that
+ * of the step.  No reference counterpart. */
+int pbf_time_kernel(pbf_sim *sim, const char *cls, uint32_t reps, double *ms);
 
 /* Measurement calibration: run a plain float4 copy of `bytes` bytes (read
  * bytes, write bytes) `reps` times on the current device and return the

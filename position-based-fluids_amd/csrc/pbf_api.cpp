@@ -192,14 +192,12 @@ struct EventTimer final : pbf::KernelTimer {
     }
     return pool[used++];
   }
-  void begin(int cls, hipStream_t st) override {
+  void events(int cls, hipEvent_t &start, hipEvent_t &stop) override {
     Rec r{cls, take(), nullptr};
-    PBF_HIP(hipEventRecord(r.a, st));
+    r.b = take();
     recs.push_back(r);
-  }
-  void end(int, hipStream_t st) override {
-    recs.back().b = take();
-    PBF_HIP(hipEventRecord(recs.back().b, st));
+    start = r.a;
+    stop = r.b;
   }
   ~EventTimer() override {
     for (auto e : pool) (void)hipEventDestroy(e);
@@ -878,6 +876,54 @@ int pbf_time_steps(pbf_sim *S, uint32_t nsteps, double *ms) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms = e;
+  });
+}
+
+int pbf_time_kernel(pbf_sim *S, const char *cls, uint32_t reps, double *ms) {
+  return guard([&] {
+    need(S && cls && ms && reps >= 1, "bad argument");
+    *ms = 0.0;
+    if (S->shape.slots == 0) return;
+    int c = -1;
+    for (int k = 0; k < pbf::kNumClasses; ++k)
+      if (std::strcmp(cls, pbf::kernel_class_name(k)) == 0) c = k;
+    need(c >= 0, "unknown kernel class");
+    PBF_HIP(hipSetDevice(S->device));
+    S->upload_scenes();
+    ensure_graph(*S);
+    PBF_HIP(hipGraphLaunch(S->graph, S->stream));  // fresh temporaries
+    // the launches are captured into a graph, as in a step
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    auto cleanup = [&] {
+      if (ge) (void)hipGraphExecDestroy(ge);
+      if (g) (void)hipGraphDestroy(g);
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    };
+    try {
+      PBF_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+      const bool ok =
+          pbf::enqueue_repeat(S->d, S->shape, S->stream, c, (int)reps);
+      PBF_HIP(hipStreamEndCapture(S->stream, &g));
+      need(ok, "kernel class has no repeatable kernel (search, lambda, delta)");
+      PBF_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      PBF_HIP(hipGraphLaunch(ge, S->stream));  // warm-up
+      PBF_HIP(hipEventCreate(&a));
+      PBF_HIP(hipEventCreate(&b));
+      PBF_HIP(hipEventRecord(a, S->stream));
+      PBF_HIP(hipGraphLaunch(ge, S->stream));
+      PBF_HIP(hipEventRecord(b, S->stream));
+      PBF_HIP(hipEventSynchronize(b));
+      float e = 0.0f;
+      PBF_HIP(hipEventElapsedTime(&e, a, b));
+      *ms = e / reps;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
   });
 }
 

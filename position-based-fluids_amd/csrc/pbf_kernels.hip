@@ -22,6 +22,7 @@
 //                  xsph_omega + vort_final.
 #include "pbf_kernels.hpp"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
@@ -1300,21 +1301,24 @@ struct Launcher {
   const StepShape &sh;
   hipStream_t st;
   KernelTimer *timer;
-  void begin(int cls) {
-    if (timer) timer->begin(cls, st);
-  }
-  void end(int cls) {
-    if (timer) timer->end(cls, st);
-  }
 };
 
 }  // namespace
 
-#define PBF_LAUNCH_SMEM(L, cls, kern, grid, block, smem, ...)      \
-  do {                                                             \
-    (L).begin(cls);                                                \
-    kern<<<(grid), (block), (smem), (L).st>>>(__VA_ARGS__);        \
-    (L).end(cls);                                                  \
+// With a timer, the kernel is launched with start/stop events attached to
+// its dispatch (hipExtLaunchKernelGGL), which time the kernel itself rather
+// than the gaps between launches.
+#define PBF_LAUNCH_SMEM(L, cls, kern, grid, block, smem, ...)                \
+  do {                                                                       \
+    if ((L).timer) {                                                         \
+      hipEvent_t ev_a_, ev_b_;                                               \
+      (L).timer->events(cls, ev_a_, ev_b_);                                  \
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block),                   \
+                            (std::uint32_t)(smem), (L).st, ev_a_, ev_b_, 0,  \
+                            __VA_ARGS__);                                    \
+    } else {                                                                 \
+      kern<<<(grid), (block), (smem), (L).st>>>(__VA_ARGS__);                \
+    }                                                                        \
   } while (0)
 #define PBF_LAUNCH(L, cls, kern, grid, block, ...) \
   PBF_LAUNCH_SMEM(L, cls, kern, grid, block, 0, __VA_ARGS__)
@@ -1420,6 +1424,45 @@ void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
     step_ref(d, L);
   else
     step_paper(d, L);
+}
+
+// reps back-to-back launches of the representative kernel of class cls
+// (search, lambda, delta), on the buffers a step left behind.  These
+// kernels write only step temporaries (lists, (x, lambda), rho, x after a
+// delta pass) that the next step rebuilds from the state before reading,
+// and none writes what it reads, so the state is unchanged.  The values
+// they compute from a previous pass's temporaries are not used.  Returns
+// false for other classes.
+bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
+                    int cls, int reps) {
+  Launcher L{sh, st, nullptr};
+  const int P = sh.slots;
+  const float4 *XS = d.xa;
+  if (cls != kClsLambda && cls != kClsDelta &&
+      !(cls == kClsSearch && sh.physics == kPaper))
+    return false;
+  for (int r = 0; r < reps; ++r) {
+    if (sh.physics == kRef) {
+      if (cls == kClsLambda)
+        PBF_GATHER(L, kClsLambda, k_lambda_ref, 0, d.scenes, d.block_scene,
+                   d.xa, d.cell_start, d.lam, d.rho);
+      else
+        PBF_GATHER(L, kClsDelta, k_delta_ref<false>, 0, d.scenes,
+                   d.block_scene, d.xa, d.cell_start, d.lam, d.rho, d.xold,
+                   d.xb, d.v);
+    } else if (cls == kClsSearch) {
+      PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene,
+                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, P);
+    } else if (cls == kClsLambda) {
+      PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
+                 XS, d.xc, d.cell_start, d.nl, d.ncnt, P, d.xb, d.rho);
+    } else {
+      PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
+                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
+                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
+    }
+  }
+  return true;
 }
 
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {

@@ -43,15 +43,17 @@ struct StepShape {
   int block[kNumClasses];  // work-group size (64, 128 or 256) per class
 };
 
-// Per-kernel-class event timing (pbf_profile_steps).
+// Per-kernel-class event timing (pbf_profile_steps, the autotuner): hands
+// out the start/stop event pair of the next launch of class cls.
 struct KernelTimer {
-  virtual void begin(int cls, hipStream_t st) = 0;
-  virtual void end(int cls, hipStream_t st) = 0;
+  virtual void events(int cls, hipEvent_t &start, hipEvent_t &stop) = 0;
   virtual ~KernelTimer() = default;
 };
 
 void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   KernelTimer *timer);
+bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
+                    int cls, int reps);
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st);
 void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   double *out_dev);

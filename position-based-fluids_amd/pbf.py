@@ -127,6 +127,8 @@ _SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_double), _P(ctypes.c_uint32),
       _P(ctypes.c_char_p), _P(ctypes.c_uint32)]),
     ("pbf_time_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_double)]),
+    ("pbf_time_kernel", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32, _P(ctypes.c_double)]),
     ("pbf_get_config", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32)]),
     ("pbf_calibrate_copy", ctypes.c_int,
      [ctypes.c_uint64, ctypes.c_uint32, _P(ctypes.c_double)]),
@@ -293,6 +295,14 @@ class Simulation:
     def time_steps(self, nsteps: int) -> float:
         ms = ctypes.c_double()
         _check(self._lib.pbf_time_steps(self._h, nsteps, ctypes.byref(ms)))
+        return ms.value
+
+    def time_kernel(self, cls: str, reps: int = 20) -> float:
+        """Average ms of one launch of kernel class `cls` (search, lambda,
+        delta), timed over `reps` back-to-back launches in a graph."""
+        ms = ctypes.c_double()
+        _check(self._lib.pbf_time_kernel(self._h, cls.encode(), reps,
+                                         ctypes.byref(ms)))
         return ms.value
 
     def config(self):
