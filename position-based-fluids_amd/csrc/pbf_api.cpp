@@ -312,9 +312,10 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     S.scenes.push_back(sc);
   }
   S.total = (int)ids;
-  // PAPER gathers address float4 arrays with 32-bit byte offsets
-  if (params[0].physics == PBF_PHYSICS_PAPER && slot > INT32_MAX / 16)
-    invalid("problem too large for PAPER physics (more than 2^27 slots)");
+  // PAPER gathers address arrays of up to 32-byte records with 32-bit byte
+  // offsets
+  if (params[0].physics == PBF_PHYSICS_PAPER && slot > INT32_MAX / 32)
+    invalid("problem too large for PAPER physics (more than 2^26 slots)");
   const pbf_params &p0 = params[0];
   pbf::StepShape &sh = S.shape;
   sh.slots = (int)slot;
@@ -356,7 +357,10 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   d.pid_s = S.alloc<int>(P);
   d.xa = S.alloc<float4>(P);
   d.xb = S.alloc<float4>(P);
-  if (sh.physics == pbf::kPaper) d.xc = S.alloc<float4>(P);
+  if (sh.physics == pbf::kPaper) {
+    d.xc = S.alloc<float4>(P);
+    d.xv = S.alloc<pbf::XVRec>(P);
+  }
   d.xold = S.alloc<float4>(P);
   d.lam = S.alloc<float>(P);
   d.rho = S.alloc<float>(P);

@@ -89,11 +89,18 @@ struct Gather {
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), (short)0,
                                             n * (int)sizeof(T), 0x00020000)) {}
   __device__ __forceinline__ T at_bytes(int off) const {
-    if constexpr (sizeof(T) == 16) {
+    if constexpr (sizeof(T) == 32) {
+      T t;
+      t.x = __builtin_bit_cast(
+          float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+      t.v = __builtin_bit_cast(
+          float4, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0));
+      return t;
+    } else if constexpr (sizeof(T) == 16) {
       return __builtin_bit_cast(
           T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
     } else {
-      static_assert(sizeof(T) == 8, "16- or 8-byte records");
+      static_assert(sizeof(T) == 8, "32-, 16- or 8-byte records");
       return __builtin_bit_cast(
           T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
@@ -671,7 +678,7 @@ __device__ __forceinline__ void for_neighbours(
     const Gather<T> &X, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, int stride, int i, float3 xs, int *rb,
     F &&f) {
-  constexpr int kShift = sizeof(T) == 16 ? 4 : 3;  // bytes per record, log2
+  constexpr int kShift = sizeof(T) == 32 ? 5 : sizeof(T) == 16 ? 4 : 3;
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
     // rb[row * bs + lane] = byte offset of the row's list base in X (lanes
@@ -955,8 +962,8 @@ __global__ __launch_bounds__(256) void k_lambda_list_h(
   ls.store(sc, i, xi, B, rho);
 }
 
-// eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST also forms
-// v = (x - x_n)/dt and carries rho in .w for XSPH.  WRITE_H (mixed
+// eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST writes
+// XV = ((x, rho), (v, m)) instead, v = (x - x_n)/dt, for the XSPH pass.  WRITE_H (mixed
 // precision) also writes the FP16 bin-relative copy for the next FP16 lambda
 // pass.
 template <bool LAST, bool WRITE_H>
@@ -966,7 +973,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
     const int *__restrict__ cs, const float *__restrict__ rho,
     const float4 *__restrict__ xold, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, int stride, const int *__restrict__ kc,
-    float4 *__restrict__ XC, float4 *__restrict__ V,
+    float4 *__restrict__ XC, XVRec *__restrict__ XV,
     uint2 *__restrict__ XH) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
@@ -1009,12 +1016,14 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   const float c = -sc.spiky * sc.inv_rho0;
   const float3 dp = make_float3(ax * c, ay * c, az * c);
   const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
-  XC[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
   if constexpr (WRITE_H) XH[i] = pack_rel(sc, xn, kc[i]);
   if constexpr (LAST) {
     const float4 xo = xold[i];
     const float3 v = velocity_exact(sc, xn, xo);
-    V[i] = make_float4(v.x, v.y, v.z, xo.w);
+    XV[i] = XVRec{make_float4(xn.x, xn.y, xn.z, rho[i]),
+                  make_float4(v.x, v.y, v.z, xo.w)};
+  } else {
+    XC[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
   }
 }
 
@@ -1022,9 +1031,9 @@ __global__ __launch_bounds__(256) void k_delta_paper(
 // without vorticity).
 __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
-    const int *__restrict__ cs, const float4 *__restrict__ V,
-    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
+    const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
+    const int *__restrict__ cs, const uint4 *__restrict__ nl,
+    const int *__restrict__ ncnt, int stride,
     const int *__restrict__ pid_s, float4 *__restrict__ pos,
     float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err) {
   extern __shared__ int rb_lds[];
@@ -1032,19 +1041,21 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i], x4 = XC[i];
+  const float4 s4 = XS[i];
+  const XVRec own = XV[i];
   const float3 xs = make_float3(s4.x, s4.y, s4.z);
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  const float4 vi = V[i];
+  const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
+  const float4 vi = own.v;
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
   const float h2 = sc.h2;
-  const Gather<float4> xsg(XS, stride), xcg(XC, stride), vg(V, stride);
-  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, stride, i, xs, rb_lds,
-                 [&](int j, int, float4 xj) {
+  const Gather<float4> xsg(XS, stride);
+  const Gather<XVRec> xvg(XV, stride);
+  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int, int, XVRec nb) {
+    const float4 xj = nb.x, vj = nb.v;
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
-      const float4 vj = vg[j];
       const float t = h2 - r2;
       // a neighbour with no density (no pair within h in the last lambda
       // pass) carries no weight
@@ -1064,9 +1075,9 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
 // N_i; writes B = (x, |omega|) for the vorticity pass.
 __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
-    const int *__restrict__ cs, const float4 *__restrict__ V,
-    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
+    const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
+    const int *__restrict__ cs, const uint4 *__restrict__ nl,
+    const int *__restrict__ ncnt, int stride,
     float4 *__restrict__ VX, float4 *__restrict__ OM,
     float4 *__restrict__ B) {
   extern __shared__ int rb_lds[];
@@ -1074,19 +1085,21 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i], x4 = XC[i];
+  const float4 s4 = XS[i];
+  const XVRec own = XV[i];
   const float3 xs = make_float3(s4.x, s4.y, s4.z);
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  const float4 vi = V[i];
+  const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
+  const float4 vi = own.v;
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
   const float h = sc.h, h2 = sc.h2;
-  const Gather<float4> xsg(XS, stride), xcg(XC, stride), vg(V, stride);
-  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, stride, i, xs, rb_lds,
-                 [&](int j, int, float4 xj) {
+  const Gather<float4> xsg(XS, stride);
+  const Gather<XVRec> xvg(XV, stride);
+  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, stride, i, xs, rb_lds,
+                 [&](int, int, XVRec nb) {
+    const float4 xj = nb.x, vj = nb.v;
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     if (r2 > 0.0f && r2 < h2) {
-      const float4 vj = vg[j];
       const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
       const float t = h2 - r2;
       const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
@@ -1393,26 +1406,26 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
     if (it == sh.iterations - 1) {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<true, false>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
+                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
     } else if (fp16(it + 1)) {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<false, true>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
+                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
     } else {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
+                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
     }
   }
   if (sh.vorticity) {
     PBF_GATHER(L, kClsXsph, k_xsph_omega, 9, d.scenes, d.block_scene, XS,
-               d.xc, d.cell_start, d.v, d.nl, d.ncnt, P, d.vx, d.om, d.xb);
+               d.xv, d.cell_start, d.nl, d.ncnt, P, d.vx, d.om, d.xb);
     PBF_GATHER(L, kClsVort, k_vort_final, 9, d.scenes, d.block_scene, XS,
                d.xb, d.cell_start, d.vx, d.om, d.nl, d.ncnt, P, d.pid_s,
                d.pos, d.vel, d.pid, d.err);
   } else {
     PBF_GATHER(L, kClsXsph, k_xsph_final_paper, 9, d.scenes, d.block_scene,
-               XS, d.xc, d.cell_start, d.v, d.nl, d.ncnt, P, d.pid_s, d.pos,
+               XS, d.xv, d.cell_start, d.nl, d.ncnt, P, d.pid_s, d.pos,
                d.vel, d.pid, d.err);
   }
 }
@@ -1459,7 +1472,7 @@ bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
     } else {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.v, d.xh);
+                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
     }
   }
   return true;

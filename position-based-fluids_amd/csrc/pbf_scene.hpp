@@ -59,6 +59,13 @@ struct Scene {
 };
 static_assert(sizeof(Scene) % 16 == 0, "Scene must stay 16-byte aligned");
 
+// Position and velocity of one slot side by side, so that a gather of a
+// neighbour's pair reads one 32-byte record.
+struct XVRec {
+  float4 x;  // x, rho
+  float4 v;  // v, m
+};
+
 // Device buffers of one handle.  Slot-indexed arrays have P entries (all
 // scenes, padded); cell-indexed arrays M (+1) entries.  "Sorted" arrays are
 // in (cell, particle id) order of the current step.
@@ -84,6 +91,7 @@ struct DevBuffers {
   float *lam;     // lambda (sorted)
   float *rho;     // density of the last lambda pass (sorted)
   float4 *v;      // velocity after the constraint solve (sorted), .w = m
+  XVRec *xv;      // PAPER: (x, rho), (v, m) after the last delta pass
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
   uint2 *xh;      // mixed precision: FP16 x - bin origin (sorted)
