@@ -376,6 +376,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   if (sh.physics == pbf::kPaper) {
     d.nl = S.alloc<uint4>(P * (size_t)(pbf::kMaxHits / 8));
     d.ncnt = S.alloc<int>(P);
+    d.rbase = S.alloc<int>(P * 9);
   }
   d.nbr = S.alloc<int>(P);
   d.red = S.alloc<double>((P / 64 + 1) * 8);

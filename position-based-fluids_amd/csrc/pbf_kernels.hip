@@ -568,8 +568,9 @@ __global__ __launch_bounds__(256) void k_reorder(
 // searches and records N_i in visiting order; the other passes replay it.
 //
 // An entry is 16 bits: stencil row (dz, dy) in the top 4 bits, offset of the
-// slot from the row's list base in the low 12 (rows and bases are those of
-// x*_i, kListWindow, recomputed by the replaying pass from the cells alone).  Entries are
+// slot from the row's list base in the low 12 (rows are those of x*_i; the
+// base of a row is the first slot of fine cell max(fx - kListWindow, 0), and
+// the search stores the 9 bases of each particle for the replaying passes).  Entries are
 // packed eight to a 16-byte group in registers (a 128-bit shift chain) and
 // each full group is one store; group g of slot i is at nl[g * P + i], so
 // lanes store and load adjacent groups.  A particle with more than kMaxHits
@@ -671,32 +672,24 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
 // f(j, row, X[j]) for every j of N_i in recorded order, eight entries
 // resolved and their eight loads issued before any is used; the padding of
 // the last group names i itself, which every pass's 0 < r^2 test (or
-// j != i) drops.  xs = x*_i; rb is LDS scratch of 9 x blockDim.x ints.
+// j != i) drops.  rbase holds the 9 row bases the search stored; rb is LDS
+// scratch of 9 x blockDim.x ints.
 template <typename T, typename F>
 __device__ __forceinline__ void for_neighbours(
     const Scene &sc, const int *__restrict__ cs, const Gather<float4> &XS,
     const Gather<T> &X, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt, int stride, int i, float3 xs, int *rb,
-    F &&f) {
+    const int *__restrict__ ncnt, const int *__restrict__ rbase, int stride,
+    int i, int *rb, F &&f) {
   constexpr int kShift = sizeof(T) == 32 ? 5 : sizeof(T) == 16 ? 4 : 3;
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
     // rb[row * bs + lane] = byte offset of the row's list base in X (lanes
-    // of one row in consecutive banks)
+    // of one row in consecutive banks), from the bases the search stored
     const int bs = blockDim.x;
     int *my = rb + threadIdx.x;
-    int cx, cy, cz;
-    cell_coords<kPaper>(sc, xs.x, xs.y, xs.z, cx, cy, cz);
-    const int win = list_window(sc, cx);
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
-      int base = 0;
-      if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1])
-        base = cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] +
-               sc.shift;
-      my[r * bs] = base << kShift;
-    }
+    for (int r = 0; r < 9; ++r)
+      my[r * bs] = rbase[(size_t)r * stride + i] << kShift;
     for (int g = 0; g * 8 < cnt; ++g) {
       const uint4 w = nl[(size_t)g * stride + i];
       const unsigned ws[4] = {w.x, w.y, w.z, w.w};
@@ -715,6 +708,8 @@ __device__ __forceinline__ void for_neighbours(
     }
   } else {
     const float h2 = sc.h2;
+    const float4 s4 = XS[i];
+    const float3 xs = make_float3(s4.x, s4.y, s4.z);
     walk_rows(sc, cs, XS, xs, [&](int j, float4 sj, int row, int) {
       const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
@@ -863,7 +858,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const int *__restrict__ cs,
     float4 *__restrict__ B, float *__restrict__ rho, uint4 *__restrict__ nl,
-    int *__restrict__ ncnt, int stride) {
+    int *__restrict__ ncnt, int *__restrict__ rbase, int stride) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
@@ -882,14 +877,23 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
       ls.add(h, h2, rx, ry, rz, r2);
     }
   });
-  // list base of the particle's own row (stencil row 4), for the padding
+  // the list base of every stencil row, for the replaying passes (row 4,
+  // the particle's own, also for the padding)
   int cx, cy, cz;
   cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
+  const int win = list_window(sc, cx);
   int self_off = -1;
-  if (cy >= 0 && cy < sc.nc[1] && cz >= 0 && cz < sc.nc[2])
-    self_off = i - (cs[sc.cell_base + (cz * sc.nc[1] + cy) * sc.nc[0] +
-                       list_window(sc, cx)] +
-                    sc.shift);
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
+    int base = 0;
+    if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1]) {
+      base = cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] +
+             sc.shift;
+      if (r == 4) self_off = i - base;
+    }
+    rbase[(size_t)r * stride + i] = base;
+  }
   rec.flush(nl, ncnt, stride, i, self_off);
   ls.store(sc, i, xi, B, rho);
 }
@@ -899,20 +903,20 @@ __global__ __launch_bounds__(256) void k_lambda_list(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ XC,
     const int *__restrict__ cs, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt, int stride, float4 *__restrict__ B,
+    const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride, float4 *__restrict__ B,
     float *__restrict__ rho) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i], x4 = XC[i];
-  const float3 xs = make_float3(s4.x, s4.y, s4.z);
+  const float4 x4 = XC[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
   const Gather<float4> xsg(XS, stride), xcg(XC, stride);
-  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, stride, i, xs, rb_lds,
+  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, rbase, stride, i, rb_lds,
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
@@ -928,7 +932,8 @@ __global__ __launch_bounds__(256) void k_lambda_list_h(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ XC,
     const uint2 *__restrict__ XH, const int *__restrict__ cs,
-    const uint4 *__restrict__ nl, const int *__restrict__ ncnt, int stride,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride,
     float4 *__restrict__ B, float *__restrict__ rho) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
@@ -948,7 +953,7 @@ __global__ __launch_bounds__(256) void k_lambda_list_h(
   const float h = sc.h, h2 = sc.h2;
   const Gather<float4> xsg(XS, stride);
   const Gather<uint2> xhg(XH, stride);
-  for_neighbours(sc, cs, xsg, xhg, nl, ncnt, stride, i, xs, rb_lds,
+  for_neighbours(sc, cs, xsg, xhg, nl, ncnt, rbase, stride, i, rb_lds,
                  [&](int j, int row, uint2 w) {
     const int dz = (row * 11) >> 5;  // row / 3 for row < 9
     const int dy = row - 3 * dz;
@@ -972,7 +977,8 @@ __global__ __launch_bounds__(256) void k_delta_paper(
     const float4 *__restrict__ XS, const float4 *__restrict__ B,
     const int *__restrict__ cs, const float *__restrict__ rho,
     const float4 *__restrict__ xold, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt, int stride, const int *__restrict__ kc,
+    const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride, const int *__restrict__ kc,
     float4 *__restrict__ XC, XVRec *__restrict__ XV,
     uint2 *__restrict__ XH) {
   extern __shared__ int rb_lds[];
@@ -980,15 +986,14 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i], x4 = B[i];
-  const float3 xs = make_float3(s4.x, s4.y, s4.z);
+  const float4 x4 = B[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float li = x4.w;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
   const int n = sc.scorr_n;
   const Gather<float4> xsg(XS, stride), bg(B, stride);
-  for_neighbours(sc, cs, xsg, bg, nl, ncnt, stride, i, xs, rb_lds,
+  for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
@@ -1033,7 +1038,8 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
     const int *__restrict__ cs, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt, int stride,
+    const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride,
     const int *__restrict__ pid_s, float4 *__restrict__ pos,
     float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err) {
   extern __shared__ int rb_lds[];
@@ -1041,16 +1047,14 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i];
   const XVRec own = XV[i];
-  const float3 xs = make_float3(s4.x, s4.y, s4.z);
   const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
   const float4 vi = own.v;
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
   const float h2 = sc.h2;
   const Gather<float4> xsg(XS, stride);
   const Gather<XVRec> xvg(XV, stride);
-  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, stride, i, xs, rb_lds,
+  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, rbase, stride, i, rb_lds,
                  [&](int, int, XVRec nb) {
     const float4 xj = nb.x, vj = nb.v;
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -1077,7 +1081,8 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
     const int *__restrict__ cs, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt, int stride,
+    const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride,
     float4 *__restrict__ VX, float4 *__restrict__ OM,
     float4 *__restrict__ B) {
   extern __shared__ int rb_lds[];
@@ -1085,16 +1090,14 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i];
   const XVRec own = XV[i];
-  const float3 xs = make_float3(s4.x, s4.y, s4.z);
   const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
   const float4 vi = own.v;
   float sx = 0.0f, sy = 0.0f, sz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
   const float h = sc.h, h2 = sc.h2;
   const Gather<float4> xsg(XS, stride);
   const Gather<XVRec> xvg(XV, stride);
-  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, stride, i, xs, rb_lds,
+  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, rbase, stride, i, rb_lds,
                  [&](int, int, XVRec nb) {
     const float4 xj = nb.x, vj = nb.v;
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -1133,7 +1136,8 @@ __global__ __launch_bounds__(256) void k_vort_final(
     const float4 *__restrict__ XS, const float4 *__restrict__ B,
     const int *__restrict__ cs, const float4 *__restrict__ VX,
     const float4 *__restrict__ OM, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt, int stride, const int *__restrict__ pid_s,
+    const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride, const int *__restrict__ pid_s,
     float4 *__restrict__ pos, float4 *__restrict__ vel, int *__restrict__ pid,
     int *__restrict__ err) {
   extern __shared__ int rb_lds[];
@@ -1141,14 +1145,13 @@ __global__ __launch_bounds__(256) void k_vort_final(
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i], x4 = B[i];
-  const float3 xs = make_float3(s4.x, s4.y, s4.z);
+  const float4 x4 = B[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 wi = OM[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   const float h = sc.h, h2 = sc.h2;
   const Gather<float4> xsg(XS, stride), bg(B, stride);
-  for_neighbours(sc, cs, xsg, bg, nl, ncnt, stride, i, xs, rb_lds,
+  for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
@@ -1395,37 +1398,37 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
   for (int it = 0; it < sh.iterations; ++it) {
     if (it == 0) {
       PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene,
-                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, P);
+                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, d.rbase, P);
     } else if (fp16(it)) {
       PBF_GATHER(L, kClsLambda, k_lambda_list_h, 9, d.scenes, d.block_scene,
-                 XS, d.xc, d.xh, d.cell_start, d.nl, d.ncnt, P, d.xb, d.rho);
+                 XS, d.xc, d.xh, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
     } else {
       PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
-                 XS, d.xc, d.cell_start, d.nl, d.ncnt, P, d.xb, d.rho);
+                 XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
     }
     if (it == sh.iterations - 1) {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<true, false>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
+                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
     } else if (fp16(it + 1)) {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<false, true>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
+                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
     } else {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
+                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
     }
   }
   if (sh.vorticity) {
     PBF_GATHER(L, kClsXsph, k_xsph_omega, 9, d.scenes, d.block_scene, XS,
-               d.xv, d.cell_start, d.nl, d.ncnt, P, d.vx, d.om, d.xb);
+               d.xv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx, d.om, d.xb);
     PBF_GATHER(L, kClsVort, k_vort_final, 9, d.scenes, d.block_scene, XS,
-               d.xb, d.cell_start, d.vx, d.om, d.nl, d.ncnt, P, d.pid_s,
+               d.xb, d.cell_start, d.vx, d.om, d.nl, d.ncnt, d.rbase, P, d.pid_s,
                d.pos, d.vel, d.pid, d.err);
   } else {
     PBF_GATHER(L, kClsXsph, k_xsph_final_paper, 9, d.scenes, d.block_scene,
-               XS, d.xv, d.cell_start, d.nl, d.ncnt, P, d.pid_s, d.pos,
+               XS, d.xv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.pid_s, d.pos,
                d.vel, d.pid, d.err);
   }
 }
@@ -1465,14 +1468,14 @@ bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                    d.xb, d.v);
     } else if (cls == kClsSearch) {
       PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene,
-                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, P);
+                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, d.rbase, P);
     } else if (cls == kClsLambda) {
       PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
-                 XS, d.xc, d.cell_start, d.nl, d.ncnt, P, d.xb, d.rho);
+                 XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
     } else {
       PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
                  d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, P, d.kc, d.xc, d.xv, d.xh);
+                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
     }
   }
   return true;
@@ -1493,7 +1496,7 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
                                               d.cell_start, d.nbr);
     k_lambda_paper<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                         d.cell_start, d.xb, d.rho, d.nl,
-                                        d.ncnt, sh.slots);
+                                        d.ncnt, d.rbase, sh.slots);
     k_unpack_lambda<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xb,
                                          d.lam);
   }

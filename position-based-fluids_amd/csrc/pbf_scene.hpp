@@ -98,6 +98,7 @@ struct DevBuffers {
   int *kc;        // mixed precision: cell id of each sorted slot
   uint4 *nl;      // hit lists: kMaxHits/8 groups of 8 x 16-bit entries per slot
   int *ncnt;      // hits per slot (> kMaxHits: list overflowed)
+  int *rbase;     // PAPER: list base slot of each stencil row, [row * P + i]
   int *nbr;       // probe: neighbour counts (sorted)
   double *red;    // diagnostics partials
   int *err;       // device error word
