@@ -589,17 +589,17 @@ struct HitRecorder {
     w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
     w3 = __builtin_amdgcn_alignbit(e, w3, 16);
   }
+  // No branch on the capacity: past kMaxHits the groups overwrite the last
+  // one, which the overflow mark (count kMaxHits + 1) makes unused.
   __device__ __forceinline__ void add(int j, int row, int b,
                                      uint4 *__restrict__ nl, int stride,
                                      int i) {
     const int off = j - b;
-    if (c < kMaxHits && off < 4096) {
-      push((unsigned)((row << 12) | off));
-      if ((c & 7) == 7)
-        nl[(size_t)(c >> 3) * stride + i] = make_uint4(w0, w1, w2, w3);
-    } else {
-      over = true;
-    }
+    over = over || off >= 4096;
+    push((unsigned)((row << 12) | (off & 0xfff)));
+    if ((c & 7) == 7)
+      nl[(size_t)min(c >> 3, kMaxHits / 8 - 1) * stride + i] =
+          make_uint4(w0, w1, w2, w3);
     ++c;
   }
   // Store the last partial group and the count.  The group is padded with
@@ -611,7 +611,7 @@ struct HitRecorder {
                                        int *__restrict__ ncnt, int stride,
                                        int i, int self_off) {
     if ((c & 7) && (self_off < 0 || self_off >= 4096)) over = true;
-    if (over) {
+    if (over || c > kMaxHits) {
       ncnt[i] = kMaxHits + 1;
       return;
     }
