@@ -364,7 +364,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   d.xold = S.alloc<float4>(P);
   d.lam = S.alloc<float>(P);
   d.rho = S.alloc<float>(P);
-  d.v = S.alloc<float4>(P);
+  if (sh.physics == pbf::kRef) d.v = S.alloc<float4>(P);
   if (sh.vorticity) {
     d.vx = S.alloc<float4>(P);
     d.om = S.alloc<float4>(P);

@@ -90,7 +90,7 @@ struct DevBuffers {
   float4 *xold;   // x at step start, m (sorted)
   float *lam;     // lambda (sorted)
   float *rho;     // density of the last lambda pass (sorted)
-  float4 *v;      // velocity after the constraint solve (sorted), .w = m
+  float4 *v;      // REF: velocity after the constraint solve (sorted), .w = m
   XVRec *xv;      // PAPER: (x, rho), (v, m) after the last delta pass
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
