@@ -893,6 +893,8 @@ int pbf_time_kernel(pbf_sim *S, const char *cls, uint32_t reps, double *ms) {
     for (int k = 0; k < pbf::kNumClasses; ++k)
       if (std::strcmp(cls, pbf::kernel_class_name(k)) == 0) c = k;
     need(c >= 0, "unknown kernel class");
+    need(pbf::repeatable_class(S->shape, c),
+         "kernel class has no repeatable kernel (search, lambda, delta)");
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
     ensure_graph(*S);
@@ -909,10 +911,8 @@ int pbf_time_kernel(pbf_sim *S, const char *cls, uint32_t reps, double *ms) {
     };
     try {
       PBF_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-      const bool ok =
-          pbf::enqueue_repeat(S->d, S->shape, S->stream, c, (int)reps);
+      pbf::enqueue_repeat(S->d, S->shape, S->stream, c, (int)reps);
       PBF_HIP(hipStreamEndCapture(S->stream, &g));
-      need(ok, "kernel class has no repeatable kernel (search, lambda, delta)");
       PBF_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
       PBF_HIP(hipGraphLaunch(ge, S->stream));  // warm-up
       PBF_HIP(hipEventCreate(&a));

@@ -1449,14 +1449,17 @@ void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
 // and none writes what it reads, so the state is unchanged.  The values
 // they compute from a previous pass's temporaries are not used.  Returns
 // false for other classes.
+bool repeatable_class(const StepShape &sh, int cls) {
+  return cls == kClsLambda || cls == kClsDelta ||
+         (cls == kClsSearch && sh.physics == kPaper);
+}
+
 bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                     int cls, int reps) {
   Launcher L{sh, st, nullptr};
   const int P = sh.slots;
   const float4 *XS = d.xa;
-  if (cls != kClsLambda && cls != kClsDelta &&
-      !(cls == kClsSearch && sh.physics == kPaper))
-    return false;
+  if (!repeatable_class(sh, cls)) return false;
   for (int r = 0; r < reps; ++r) {
     if (sh.physics == kRef) {
       if (cls == kClsLambda)

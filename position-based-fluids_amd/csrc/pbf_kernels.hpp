@@ -52,6 +52,7 @@ struct KernelTimer {
 
 void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   KernelTimer *timer);
+bool repeatable_class(const StepShape &sh, int cls);
 bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                     int cls, int reps);
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st);

@@ -430,3 +430,23 @@ def test_profile_and_timing_hooks():
         assert all(ms >= 0 for ms, _ in prof.values())
         assert s.time_steps(3) > 0
         assert s.config()[0] in (64, 128, 256)
+
+
+def test_time_kernel_leaves_the_state_of_one_step():
+    """pbf_time_kernel runs one step, then re-launches one kernel class; the
+    state must be that of the step (the re-launched kernels write step
+    temporaries only)."""
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=3)
+    p, parts = sc.params[0], sc.particles[0]
+    say("time_kernel state")
+    ref = run_gpu(p, parts, 4)
+    with pbf.Simulation(p, parts) as s:
+        s.step(1)
+        for cls in ("search", "lambda", "delta"):
+            assert s.time_kernel(cls, 3) > 0   # each adds one step: 4 in all
+        got = s.dump()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    with pbf.Simulation(p, parts) as s:
+        with pytest.raises(pbf.PBFError):
+            s.time_kernel("xsph", 3)
