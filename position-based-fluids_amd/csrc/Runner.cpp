@@ -6,15 +6,19 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <numeric>
 #include <sstream>
 
 namespace {
 
+// The .in format of PartReader (count, then "m x y z vx vy vz" lines), with
+// 9 significant digits so that reading the file back gives the same floats.
 void writeParticles(const std::string &name, unsigned n, const Float4 *pos,
                     const Float4 *vel) {
   std::ofstream o(name.c_str());
+  o << std::setprecision(9);
   o << n << "\n";
   for (unsigned i = 0; i < n; ++i)
     o << vel[i].s[3] << " " << pos[i].s[0] << " " << pos[i].s[1] << " "
@@ -25,6 +29,7 @@ void writeParticles(const std::string &name, unsigned n, const Float4 *pos,
 void writeVtk(const std::string &name, unsigned n, const Float4 *pos,
               const Float4 *vel) {
   std::ofstream o(name.c_str());
+  o << std::setprecision(9);
   o << "# vtk DataFile Version 3.0\npbf particles\nASCII\n"
     << "DATASET POLYDATA\nPOINTS " << n << " float\n";
   for (unsigned i = 0; i < n; ++i)

@@ -2,10 +2,11 @@
 //
 // Layout: particles live in slots sorted by (cell, particle id); each
 // scene's slots start at a multiple of kSlotAlign.  A cell table
-// cell_start[M+1] (exclusive scan of per-cell counts, unpadded) gives every
-// cell's slot range, and because cell ids run x-fastest the three x-cells of
-// one (y, z) row of the 27-cell stencil are ONE contiguous slot range:
-// every gather walks 9 ranges instead of 27 lists.
+// cell_start (exclusive scan of per-cell counts) gives every cell's slot
+// range, and because cell ids run x-fastest the x-cells of one (y, z) row of
+// the stencil are ONE contiguous slot range: a search walks 9 ranges
+// instead of 27 lists.  PAPER bins x in kXSub-times finer cells, so each
+// range can be cut to the particle's reach.
 //
 // Step (reference sequence src/Simulation.cpp:455-576):
 //   predict_count  predict (predict_positions.cl:5-9), cell id
@@ -14,12 +15,16 @@
 //   scatter        slot = cell_start + rank   (arrival order)
 //   reorder        in-cell rank by particle id -> deterministic (cell, id)
 //                  order; gathers x*, x, id into sorted slots
-//   I x (lambda, delta)   27-cell gathers (compute_scaling.cl,
-//                  compute_delta.cl + update_predicted.cl); the last delta
-//                  pass also forms v = (x* - x)/dt (update_velocities.cl)
-//   xsph_final     XSPH (apply_vorticity_and_viscosity.cl) fused with
-//                  update_positions.cl; PAPER with vorticity splits it into
-//                  xsph_omega + vort_final.
+//   REF: I x (lambda, delta), each searching the 27-cell stencil
+//                  (compute_scaling.cl, compute_delta.cl +
+//                  update_predicted.cl); the last delta pass also forms
+//                  v = (x* - x)/dt (update_velocities.cl); xsph_final: XSPH
+//                  (apply_vorticity_and_viscosity.cl) fused with
+//                  update_positions.cl.
+//   PAPER: the first lambda pass searches and records the neighbour lists
+//                  of the step; the other lambda and delta passes, XSPH +
+//                  omega and the vorticity pass (fused with
+//                  update_positions.cl) replay them.
 #include "pbf_kernels.hpp"
 
 #include <hip/hip_ext.h>
@@ -720,7 +725,7 @@ __device__ __forceinline__ void for_neighbours(
 
 // ---------------------------------------------------------------------------
 // constraint solve, REF (reference arithmetic and order; lambda array,
-// x* ping-pong)
+// x* alternating between two buffers)
 // ---------------------------------------------------------------------------
 
 // compute_scaling.cl:12-140.

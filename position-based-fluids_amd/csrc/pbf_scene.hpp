@@ -31,7 +31,7 @@ static_assert(kReachMargin * kXSub < 1.0f, "reach must stay within the window");
 
 // Capacity of a particle's neighbour list (pairs within h at x*).
 constexpr int kMaxHits = 96;
-static_assert(kMaxHits % 8 == 0, "hit lists are stored in groups of 8");
+static_assert(kMaxHits % 8 == 0, "lists are stored in groups of 8 entries");
 
 // Per-scene constants, resident in device memory and read by the kernels
 // with scalar loads (the scene index is uniform per work-group).  REF
@@ -96,8 +96,9 @@ struct DevBuffers {
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
   uint2 *xh;      // mixed precision: FP16 x - bin origin (sorted)
   int *kc;        // mixed precision: cell id of each sorted slot
-  uint4 *nl;      // hit lists: kMaxHits/8 groups of 8 x 16-bit entries per slot
-  int *ncnt;      // hits per slot (> kMaxHits: list overflowed)
+  uint4 *nl;      // PAPER neighbour lists: kMaxHits/8 groups of 8 x 16-bit
+                  // entries per slot
+  int *ncnt;      // list length per slot (> kMaxHits: list overflowed)
   int *rbase;     // PAPER: list base slot of each stencil row, [row * P + i]
   int *nbr;       // probe: neighbour counts (sorted)
   double *red;    // diagnostics partials

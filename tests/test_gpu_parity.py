@@ -170,6 +170,20 @@ def test_paper_vs_fp64_oracle():
     assert np.abs(gv[:, :3] - ov[:, :3]).max() <= 1e-3 * np.abs(ov[:, :3]).max()
 
 
+def test_paper_c0_vs_fp64_oracle():
+    """The full 32,768-particle c0 dam in PAPER physics, 10 steps.  The FP32
+    oracle itself sits 4.7e-5 m / 2.0e-3 m/s from the FP64 one here, so the
+    bars are 1e-4 m and, since v = (x* - x)/dt, 1e-4 m / dt."""
+    c0 = scenarios.config(0, physics=pbf.PHYSICS_PAPER)
+    p, parts = c0.params[0], c0.particles[0]
+    say("paper c0 vs fp64", c0.n)
+    gp, gv = run_gpu(p, parts, 10)
+    pos, vel = orc.state_from_particles(parts, np.float64)
+    op, ov = orc.run_f64(p, pos, vel, 10)
+    assert np.abs(gp[:, :3] - op[:, :3]).max() <= 1e-4
+    assert np.abs(gv[:, :3] - ov[:, :3]).max() <= 1e-4 / p.time_step_length
+
+
 def test_paper_sphere_obstacle():
     # the block rests its lowest layer on the obstacle's r + h shell and
     # falls onto it
@@ -295,6 +309,38 @@ def test_mixed_sphere_and_batch():
         for k in range(2):
             bp, _ = b.dump_scene(k)
             assert np.array_equal(bp, gp)
+
+
+# ----------------------------------------------------------------------------
+# the C++ facade (Simulation / Runner / readers) through pbf_run
+# ----------------------------------------------------------------------------
+
+def test_cpp_runner_matches_c_abi(tmp_path, golden_dir):
+    """pbf_run (ConfigReader, PartReader, Simulation, Runner) on the
+    reference's dam scenario writes .out files every 5 steps; read back,
+    they equal the C-ABI run of the same scenario bit for bit."""
+    import shutil
+    import subprocess
+    shutil.copy(os.path.join(golden_dir, "dam.in"), tmp_path / "dam.in")
+    text = open(os.path.join(golden_dir, "dam.par")).read()
+    text += "\npart_out_freq 5\npart_out_name_base out_\n"
+    (tmp_path / "dam.par").write_text(text)
+    exe = os.path.join(os.path.dirname(pbf.LIB_PATH), "pbf_run")
+    say("pbf_run dam.par --steps 10")
+    r = subprocess.run([exe, "dam.par", "--steps", "10"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "mean:" in r.stdout and "median:" in r.stdout
+    p, part_file = pbf.read_par(str(tmp_path / "dam.par"))
+    parts = pbf.read_particles(str(tmp_path / part_file))
+    with pbf.Simulation(p, parts) as s:
+        for k in (1, 2):
+            s.step(5)
+            pos, vel = s.dump()
+            got = pbf.read_particles(str(tmp_path / f"out_{k}.out"))
+            assert np.array_equal(got[:, 1:4], pos[:, :3]), k
+            assert np.array_equal(got[:, 4:7], vel[:, :3]), k
+            assert np.array_equal(got[:, 0], vel[:, 3]), k
 
 
 # ----------------------------------------------------------------------------
