@@ -319,7 +319,7 @@ def main():
     ap.add_argument("--physics", choices=["paper", "ref"], default="paper")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="steps of the 256-scene batch vs sequential "
