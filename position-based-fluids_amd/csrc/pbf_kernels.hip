@@ -31,6 +31,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace pbf {
 
@@ -60,6 +61,14 @@ __device__ __forceinline__ int trunc_int(float v) {
   if (v != v) return 0;
   v = fminf(fmaxf(v, -1e9f), 1e9f);
   return (int)v;
+}
+
+// 0 < r2 < h2 as one unsigned compare of the bit patterns: r2 is a sum of
+// squares (+0 or positive), h2 > 0 is finite, and the order of positive
+// floats is the order of their bits; +0 wraps to 0xffffffff, NaN and inf
+// compare above h2.
+__device__ __forceinline__ bool in_reach(float r2, float h2) {
+  return __float_as_uint(r2) - 1u < __float_as_uint(h2) - 1u;
 }
 
 // Mixed precision: a position as FP16 offsets from the origin of its bin
@@ -688,22 +697,29 @@ __device__ __forceinline__ void for_neighbours(
   constexpr int kShift = sizeof(T) == 32 ? 5 : sizeof(T) == 16 ? 4 : 3;
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
-    // rb[row * bs + lane] = byte offset of the row's list base in X (lanes
-    // of one row in consecutive banks), from the bases the search stored
-    const int bs = blockDim.x;
+    // rb[row * bs + lane] = the row's list base (lanes of one row in
+    // consecutive banks), from the bases the search stored
+    const unsigned bs = blockDim.x;
     int *my = rb + threadIdx.x;
 #pragma unroll
-    for (int r = 0; r < 9; ++r)
-      my[r * bs] = rbase[(size_t)r * stride + i] << kShift;
+    for (int r = 0; r < 9; ++r) my[r * bs] = rbase[(size_t)r * stride + i];
+    // table address row * 4 bs + 4 lane (one 24-bit multiply-add); record
+    // byte offset (base + entry) << shift (one add-shift)
+    const unsigned bs4 = bs * 4u, lane4 = threadIdx.x * 4u;
+    const char *rbb = reinterpret_cast<const char *>(rb);
     for (int g = 0; g * 8 < cnt; ++g) {
       const uint4 w = nl[(size_t)g * stride + i];
       const unsigned ws[4] = {w.x, w.y, w.z, w.w};
       int off[8], rows[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const unsigned e = (ws[q >> 1] >> ((q & 1) * 16)) & 0xffffu;
-        rows[q] = (int)(e >> 12);
-        off[q] = my[rows[q] * bs] + ((int)(e & 0xfffu) << kShift);
+        const unsigned word = ws[q >> 1];
+        const unsigned row = (q & 1) ? word >> 28 : (word >> 12) & 0xfu;
+        const unsigned ent = (q & 1) ? (word >> 16) & 0xfffu : word & 0xfffu;
+        rows[q] = (int)row;
+        off[q] = (*reinterpret_cast<const int *>(rbb + row * bs4 + lane4) +
+                  (int)ent)
+                 << kShift;
       }
       T xv[8];
 #pragma unroll
@@ -845,6 +861,22 @@ struct LambdaSums {
     gy += ry * gs;
     gz += rz * gs;
   }
+  // add() of a replayed entry with no branch: the terms are formed for every
+  // entry and selected to 0 for one out of reach (r = 0 gives inf and NaN
+  // there, which the selects drop); the sums are those of add().
+  __device__ __forceinline__ void add_if(bool in, float h, float h2, float rx,
+                                         float ry, float rz, float r2) {
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float hr = h - r2 * inv;
+    const float t = in ? h2 - r2 : 0.0f;
+    const float hr2 = hr * hr;
+    wsum += t * t * t;
+    g2 += in ? hr2 * hr2 : 0.0f;
+    const float gs = in ? hr2 * inv : 0.0f;
+    gx += rx * gs;
+    gy += ry * gs;
+    gz += rz * gs;
+  }
   __device__ __forceinline__ void store(const Scene &sc, int i, float3 xi,
                                         float4 *__restrict__ B,
                                         float *__restrict__ rho) const {
@@ -925,7 +957,7 @@ __global__ __launch_bounds__(256) void k_lambda_list(
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && r2 < h2) ls.add(h, h2, rx, ry, rz, r2);
+    ls.add_if(in_reach(r2, h2), h, h2, rx, ry, rz, r2);
   });
   ls.store(sc, i, xi, B, rho);
 }
@@ -998,17 +1030,20 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
   const int n = sc.scorr_n;
   const Gather<float4> xsg(XS, stride), bg(B, stride);
-  for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, float4 xj) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && r2 < h2) {
+  // (lambda_i + lambda_j + s_corr) (h-r)^2 r/|r| summed branch-free (the
+  // out-of-reach entries selected to 0); the loop is compiled twice, for
+  // the paper's exponent n = 4 and for any n (uniform per scene)
+  const auto sum = [&](auto n4) {
+    for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
+                   [&](int, int, float4 xj) {
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
       const float inv = __builtin_amdgcn_rsqf(r2);
       const float hr = h - r2 * inv;
       const float t = h2 - r2;
       const float q = t * t * t * cq;  // W / W(dq)
       float qn;
-      if (n == 4) {  // the paper's exponent (n is uniform: no divergence)
+      if constexpr (decltype(n4)::value) {
         const float q2 = q * q;
         qn = q2 * q2;
       } else {
@@ -1016,12 +1051,16 @@ __global__ __launch_bounds__(256) void k_delta_paper(
         for (int e = 0; e < n; ++e) qn *= q;
       }
       const float coef = li + xj.w - kk * qn;
-      const float gs = coef * hr * hr * inv;
+      const float gs = in_reach(r2, h2) ? coef * hr * hr * inv : 0.0f;
       ax += rx * gs;
       ay += ry * gs;
       az += rz * gs;
-    }
-  });
+    });
+  };
+  if (n == 4)
+    sum(std::true_type{});
+  else
+    sum(std::false_type{});
   // grad_i W = -spiky (h-r)^2 r/|r|
   const float c = -sc.spiky * sc.inv_rho0;
   const float3 dp = make_float3(ax * c, ay * c, az * c);
@@ -1064,15 +1103,15 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const float4 xj = nb.x, vj = nb.v;
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && r2 < h2) {
-      const float t = h2 - r2;
-      // a neighbour with no density (no pair within h in the last lambda
-      // pass) carries no weight
-      const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
-      sx += (vj.x - vi.x) * w;
-      sy += (vj.y - vi.y) * w;
-      sz += (vj.z - vi.z) * w;
-    }
+    const float t = h2 - r2;
+    // a neighbour with no density (no pair within h in the last lambda
+    // pass) carries no weight
+    const float w = in_reach(r2, h2) && xj.w > 0.0f
+                        ? t * t * t * __builtin_amdgcn_rcpf(xj.w)
+                        : 0.0f;
+    sx += (vj.x - vi.x) * w;
+    sy += (vj.y - vi.y) * w;
+    sz += (vj.z - vi.z) * w;
   });
   const float c = sc.c_xsph * sc.poly6;
   const float4 v = make_float4(vi.x + c * sx, vi.y + c * sy, vi.z + c * sz, vi.w);
@@ -1107,21 +1146,22 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
     const float4 xj = nb.x, vj = nb.v;
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && r2 < h2) {
-      const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
-      const float t = h2 - r2;
-      const float w = xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
-      sx += vx * w;
-      sy += vy * w;
-      sz += vz * w;
-      const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
-      const float gs = hr * hr * inv;  // grad_j W_ij = spiky gs r
-      const float gx = rx * gs, gy = ry * gs, gz = rz * gs;
-      ox += vy * gz - vz * gy;
-      oy += vz * gx - vx * gz;
-      oz += vx * gy - vy * gx;
-    }
+    // branch-free: the terms of an out-of-reach entry are selected to 0
+    const bool in = in_reach(r2, h2);
+    const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
+    const float t = h2 - r2;
+    const float w =
+        in && xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
+    sx += vx * w;
+    sy += vy * w;
+    sz += vz * w;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float hr = h - r2 * inv;
+    const float gs = in ? hr * hr * inv : 0.0f;  // grad_j W_ij = spiky gs r
+    const float gx = rx * gs, gy = ry * gs, gz = rz * gs;
+    ox += vy * gz - vz * gy;
+    oy += vz * gx - vx * gz;
+    oz += vx * gy - vy * gx;
   });
   const float cx = sc.c_xsph * sc.poly6;
   VX[i] = make_float4(vi.x + cx * sx, vi.y + cx * sy, vi.z + cx * sz, vi.w);
@@ -1160,14 +1200,13 @@ __global__ __launch_bounds__(256) void k_vort_final(
                  [&](int, int, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && r2 < h2) {
-      const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
-      const float gs = (xj.w - wi.w) * hr * hr * inv;
-      ex += rx * gs;
-      ey += ry * gs;
-      ez += rz * gs;
-    }
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float hr = h - r2 * inv;
+    const float gs =
+        in_reach(r2, h2) ? (xj.w - wi.w) * hr * hr * inv : 0.0f;
+    ex += rx * gs;
+    ey += ry * gs;
+    ez += rz * gs;
   });
   float4 v = VX[i];
   const float e2 = ex * ex + ey * ey + ez * ez;
