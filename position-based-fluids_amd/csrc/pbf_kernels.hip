@@ -584,33 +584,30 @@ __global__ __launch_bounds__(256) void k_reorder(
 // An entry is 16 bits: stencil row (dz, dy) in the top 4 bits, offset of the
 // slot from the row's list base in the low 12 (rows are those of x*_i; the
 // base of a row is the first slot of fine cell max(fx - kListWindow, 0), and
-// the search stores the 9 bases of each particle for the replaying passes).  Entries are
-// packed eight to a 16-byte group in registers (a 128-bit shift chain) and
-// each full group is one store; group g of slot i is at nl[g * P + i], so
-// lanes store and load adjacent groups.  A particle with more than kMaxHits
-// neighbours (or a row longer than 4096 slots) is marked with a count of
-// kMaxHits + 1; the replaying passes then walk its rows again with the
-// search's test at x*.
+// the search stores the 9 bases of each particle for the replaying passes).
+// Entries are packed eight to a 16-byte group in registers (a 128-bit shift
+// chain) and each full group is one store; group g of slot i is at
+// nl[g * P + i], so lanes store and load adjacent groups.  A particle with
+// more than kMaxHits neighbours (or a row whose candidates reach 4096 slots
+// past its list base) is marked with a count of kMaxHits + 1; the replaying
+// passes then walk its rows again with the search's test at x*.
 // ---------------------------------------------------------------------------
 
 struct HitRecorder {
   unsigned w0, w1, w2, w3;
   int c;
-  bool over;
   __device__ __forceinline__ void push(unsigned e) {
     w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
     w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
     w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
     w3 = __builtin_amdgcn_alignbit(e, w3, 16);
   }
-  // No branch on the capacity: past kMaxHits the groups overwrite the last
-  // one, which the overflow mark (count kMaxHits + 1) makes unused.
-  __device__ __forceinline__ void add(int j, int row, int b,
-                                     uint4 *__restrict__ nl, int stride,
-                                     int i) {
-    const int off = j - b;
-    over = over || off >= 4096;
-    push((unsigned)((row << 12) | (off & 0xfff)));
+  // Record an entry (its low 16 bits).  No branch on the capacity: past
+  // kMaxHits the groups overwrite the last one, which the overflow mark
+  // (count kMaxHits + 1) makes unused.
+  __device__ __forceinline__ void add(unsigned entry, uint4 *__restrict__ nl,
+                                     int stride, int i) {
+    push(entry);
     if ((c & 7) == 7)
       nl[(size_t)min(c >> 3, kMaxHits / 8 - 1) * stride + i] =
           make_uint4(w0, w1, w2, w3);
@@ -620,11 +617,12 @@ struct HitRecorder {
   // entries naming slot i itself (stencil row 4 is the particle's own row;
   // self_off = i - that row's list base, < 0 when it has none), which every
   // replaying pass drops (r = 0), so a replay decodes all eight entries of
-  // a group without a count test.
+  // a group without a count test.  wide: a row's offsets may not fit 12
+  // bits, so the list is marked as overflowed.
   __device__ __forceinline__ void flush(uint4 *__restrict__ nl,
                                        int *__restrict__ ncnt, int stride,
-                                       int i, int self_off) {
-    if ((c & 7) && (self_off < 0 || self_off >= 4096)) over = true;
+                                       int i, int self_off, bool wide) {
+    const bool over = wide || ((c & 7) && (unsigned)self_off >= 4096u);
     if (over || c > kMaxHits) {
       ncnt[i] = kMaxHits + 1;
       return;
@@ -638,23 +636,22 @@ struct HitRecorder {
   }
 };
 
-// Stencil walk of a PAPER searching pass: f(j, X[j], row, row_start) for
-// every slot of the 9 row ranges, candidate positions loaded four at a time
-// so that four loads are in flight per wait.
 // Fine x cell at which the list entries of a particle in fine cell cx start
 // counting, in every row.
 __device__ __forceinline__ int list_window(const Scene &sc, int cx) {
   return min(max(cx - kListWindow, 0), sc.nc[0] - 1);
 }
 
-// Stencil walk of a PAPER searching pass: f(j, X[j], row, list base) for
-// every slot of the 9 row ranges, candidate positions loaded four at a time
-// so that four loads are in flight per wait.
-template <typename F>
+// Stencil walk of a PAPER searching pass over the rows of the 9 that lie in
+// the grid: on_row(row, lb, b, e) once per row (lb its list base, [b, e)
+// its candidate slots), then f(j, X[j], (row << 12) - lb) for every
+// candidate, whose list entry is that value + j; candidate positions are
+// loaded four at a time so that four loads are in flight per wait.
+template <typename R, typename F>
 __device__ __forceinline__ void walk_rows(const Scene &sc,
                                           const int *__restrict__ cs,
                                           const Gather<float4> &X, float3 xi,
-                                          F &&f) {
+                                          R &&on_row, F &&f) {
   int cx, cy, cz;
   cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
   const int win = list_window(sc, cx);
@@ -664,21 +661,22 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
     for (int dy = 0; dy < 3; ++dy) {
       const int yy = cy - 1 + dy;
       if (yy < 0 || yy >= sc.nc[1]) continue;
-      int b, e;
-      row_range(sc, cs, xi, yy, zz, b, e);
-      if (b >= e) continue;
+      const int row = dz * 3 + dy;
       const int lb =
           cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] + sc.shift;
-      const int row = dz * 3 + dy;
+      int b, e;
+      row_range(sc, cs, xi, yy, zz, b, e);
+      on_row(row, lb, b, e);
+      const int re = (row << 12) - lb;
       int j = b;
       for (; j + 4 <= e; j += 4) {
         const float4 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2], a3 = X[j + 3];
-        f(j, a0, row, lb);
-        f(j + 1, a1, row, lb);
-        f(j + 2, a2, row, lb);
-        f(j + 3, a3, row, lb);
+        f(j, a0, re);
+        f(j + 1, a1, re);
+        f(j + 2, a2, re);
+        f(j + 3, a3, re);
       }
-      for (; j < e; ++j) f(j, X[j], row, lb);
+      for (; j < e; ++j) f(j, X[j], re);
     }
   }
 }
@@ -731,11 +729,14 @@ __device__ __forceinline__ void for_neighbours(
     const float h2 = sc.h2;
     const float4 s4 = XS[i];
     const float3 xs = make_float3(s4.x, s4.y, s4.z);
-    walk_rows(sc, cs, XS, xs, [&](int j, float4 sj, int row, int) {
-      const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
-      const float r2 = rx * rx + ry * ry + rz * rz;
-      if (r2 > 0.0f && r2 < h2) f(j, row, X[j]);
-    });
+    int row = 0;
+    walk_rows(
+        sc, cs, XS, xs, [&](int r, int, int, int) { row = r; },
+        [&](int j, float4 sj, int) {
+          const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          if (r2 > 0.0f && r2 < h2) f(j, row, X[j]);
+        });
   }
 }
 
@@ -902,36 +903,36 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
   if (i - sc.slot_off >= sc.n) return;
   const float4 x4 = XS[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  HitRecorder rec{0u, 0u, 0u, 0u, 0, false};
+  HitRecorder rec{0u, 0u, 0u, 0u, 0};
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
   const Gather<float4> xsg(XS, stride);
-  walk_rows(sc, cs, xsg, xi, [&](int j, float4 xj, int row, int b) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 > 0.0f && r2 < h2) {
-      rec.add(j, row, b, nl, stride, i);
-      ls.add(h, h2, rx, ry, rz, r2);
-    }
-  });
-  // the list base of every stencil row, for the replaying passes (row 4,
-  // the particle's own, also for the padding)
-  int cx, cy, cz;
-  cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
-  const int win = list_window(sc, cx);
+  // the list base of every stencil row in the grid, for the replaying
+  // passes (row 4, the particle's own, also for the padding); rows outside
+  // the grid name no entry and get 0
   int self_off = -1;
+  unsigned in_grid = 0;
+  bool wide = false;
+  walk_rows(
+      sc, cs, xsg, xi,
+      [&](int row, int lb, int b, int e) {
+        rbase[(size_t)row * stride + i] = lb;
+        in_grid |= 1u << row;
+        if (row == 4) self_off = i - lb;
+        wide = wide || e - lb > 4096;
+      },
+      [&](int j, float4 xj, int re) {
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > 0.0f && r2 < h2) {
+          rec.add((unsigned)(re + j), nl, stride, i);
+          ls.add(h, h2, rx, ry, rz, r2);
+        }
+      });
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
-    int base = 0;
-    if (zz >= 0 && zz < sc.nc[2] && yy >= 0 && yy < sc.nc[1]) {
-      base = cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] +
-             sc.shift;
-      if (r == 4) self_off = i - base;
-    }
-    rbase[(size_t)r * stride + i] = base;
-  }
-  rec.flush(nl, ncnt, stride, i, self_off);
+  for (int r = 0; r < 9; ++r)
+    if (!(in_grid >> r & 1u)) rbase[(size_t)r * stride + i] = 0;
+  rec.flush(nl, ncnt, stride, i, self_off, wide);
   ls.store(sc, i, xi, B, rho);
 }
 

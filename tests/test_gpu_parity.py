@@ -402,6 +402,23 @@ def test_all_particles_in_one_cell():
     assert np.array_equal(gp, op) and np.array_equal(gv, ov)
 
 
+@pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
+def test_nan_particle_is_reported(physics):
+    """A non-finite input position bins into cell (0, 0, 0) (the cell
+    coordinate of NaN is 0), is no particle's neighbour (r^2 is NaN) and is
+    reported by the device error word as PBF_ERR_NUMERIC."""
+    say("edge: NaN particle", physics)
+    sc = scenarios.small_dam(dims=(8, 8, 8), box=(1.0, 1.0, 1.0),
+                             physics=physics)
+    parts = sc.particles[0].copy()
+    parts[7, 1:4] = np.nan
+    with pbf.Simulation(sc.params[0], parts) as s:
+        s.step(2)
+        with pytest.raises(pbf.PBFError) as e:
+            s.synchronize()
+    assert e.value.code == pbf.ERR_NUMERIC
+
+
 # ----------------------------------------------------------------------------
 # batching, graphs, determinism, full size
 # ----------------------------------------------------------------------------
