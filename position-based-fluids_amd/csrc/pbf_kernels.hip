@@ -735,7 +735,7 @@ __device__ __forceinline__ void for_neighbours(
         [&](int j, float4 sj, int) {
           const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
           const float r2 = rx * rx + ry * ry + rz * rz;
-          if (r2 > 0.0f && r2 < h2) f(j, row, X[j]);
+          if (in_reach(r2, h2)) f(j, row, X[j]);
         });
   }
 }
@@ -924,7 +924,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
       [&](int j, float4 xj, int re) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > 0.0f && r2 < h2) {
+        if (in_reach(r2, h2)) {
           rec.add((unsigned)(re + j), nl, stride, i);
           ls.add(h, h2, rx, ry, rz, r2);
         }
@@ -1000,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_lambda_list_h(
     const float ry = (qy0 - (float)dy * sc.cell_len[1]) - half_hi(w.x);
     const float rz = (qz0 - (float)dz * sc.cell_len[2]) - half_lo(w.y);
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (j != i && r2 > 0.0f && r2 < h2) ls.add(h, h2, rx, ry, rz, r2);
+    if (j != i && in_reach(r2, h2)) ls.add(h, h2, rx, ry, rz, r2);
   });
   ls.store(sc, i, xi, B, rho);
 }
