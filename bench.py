@@ -25,14 +25,19 @@ import scenarios  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (compulsory) HBM bytes per particle per launch of each kernel
-# class, FP32.  PAPER passes stream a neighbour list of L = ceil(n/8) 16-byte
-# groups (n neighbours, 2 bytes each); "delta" is the mean over the I
-# launches (the last one also reads rho and x_n and writes v).
+# class, FP32.  PAPER passes stream a neighbour list of L = ceil(n/11)
+# 16-byte groups (n neighbours, 11 bits each); "delta" is the mean over the
+# I launches (the last one also reads rho and x_n and writes v).
+LIST_PER_GROUP = 11
+LIST_MAX = 99
+
+
 def list_bytes(counts) -> float:
     """Mean list bytes per particle for neighbour counts `counts` (lists
-    longer than 96 entries are not stored)."""
+    longer than LIST_MAX entries are not stored)."""
     c = np.asarray(counts, np.float64)
-    return float(np.mean(np.where(c <= 96, 16.0 * np.ceil(c / 8.0), 0.0)))
+    return float(np.mean(np.where(c <= LIST_MAX,
+                                  16.0 * np.ceil(c / LIST_PER_GROUP), 0.0)))
 
 
 def algorithmic_bytes(cls: str, iterations: int, physics: int,

@@ -374,7 +374,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     d.kc = S.alloc<int>(P);
   }
   if (sh.physics == pbf::kPaper) {
-    d.nl = S.alloc<uint4>(P * (size_t)(pbf::kMaxHits / 8));
+    d.nl = S.alloc<uint4>(P * (size_t)pbf::kMaxGroups);
     d.ncnt = S.alloc<int>(P);
     d.rbase = S.alloc<int>(P * 9);
   }

@@ -581,72 +581,74 @@ __global__ __launch_bounds__(256) void k_reorder(
 // own positions, a pair contributing while 0 < r < h.  The first lambda pass
 // searches and records N_i in visiting order; the other passes replay it.
 //
-// An entry is 16 bits: stencil row (dz, dy) in the top 4 bits, offset of the
-// slot from the row's list base in the low 12 (rows are those of x*_i; the
-// base of a row is the first slot of fine cell max(fx - kListWindow, 0), and
-// the search stores the 9 bases of each particle for the replaying passes).
-// Entries are packed eight to a 16-byte group in registers (a 128-bit shift
-// chain) and each full group is one store; group g of slot i is at
-// nl[g * P + i], so lanes store and load adjacent groups.  A particle with
-// more than kMaxHits neighbours (or a row whose candidates reach 4096 slots
-// past its list base) is marked with a count of kMaxHits + 1; the replaying
-// passes then walk its rows again with the search's test at x*.
+// An entry is kEntryBits bits: stencil row (dz, dy) in the top 4 bits and
+// the slot's offset from the row's first candidate slot in the low kOffBits
+// (rows are those of x*_i; the search stores each particle's 9 row starts
+// for the replaying passes).  kPerGroup entries are packed into a 16-byte
+// group in registers (a 128-bit shift chain) and each full group is one
+// store; group g of slot i is at nl[g * P + i], so lanes store and load
+// adjacent groups.  A particle with more than kMaxHits neighbours (or a row
+// of more than 2^kOffBits candidates) is marked with a count of
+// kMaxHits + 1; the replaying passes then walk its rows again with the
+// search's test at x*.
 // ---------------------------------------------------------------------------
 
 struct HitRecorder {
   unsigned w0, w1, w2, w3;
-  int c;
+  int c;     // entries recorded
+  int g;     // full groups stored
+  int next;  // value of c at which the group in the registers is full
   __device__ __forceinline__ void push(unsigned e) {
-    w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-    w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-    w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-    w3 = __builtin_amdgcn_alignbit(e, w3, 16);
+    w0 = __builtin_amdgcn_alignbit(w1, w0, kEntryBits);
+    w1 = __builtin_amdgcn_alignbit(w2, w1, kEntryBits);
+    w2 = __builtin_amdgcn_alignbit(w3, w2, kEntryBits);
+    w3 = __builtin_amdgcn_alignbit(e, w3, kEntryBits);
   }
-  // Record an entry (its low 16 bits).  No branch on the capacity: past
-  // kMaxHits the groups overwrite the last one, which the overflow mark
-  // (count kMaxHits + 1) makes unused.
+  // Record an entry (its low kEntryBits bits).  No branch on the capacity:
+  // past kMaxHits the groups overwrite the last one, which the overflow
+  // mark (count kMaxHits + 1) makes unused.
   __device__ __forceinline__ void add(unsigned entry, uint4 *__restrict__ nl,
                                      int stride, int i) {
     push(entry);
-    if ((c & 7) == 7)
-      nl[(size_t)min(c >> 3, kMaxHits / 8 - 1) * stride + i] =
+    if (c == next) {
+      nl[(size_t)min(g, kMaxGroups - 1) * stride + i] =
           make_uint4(w0, w1, w2, w3);
+      ++g;
+      next += kPerGroup;
+    }
     ++c;
   }
   // Store the last partial group and the count.  The group is padded with
   // entries naming slot i itself (stencil row 4 is the particle's own row;
-  // self_off = i - that row's list base, < 0 when it has none), which every
-  // replaying pass drops (r = 0), so a replay decodes all eight entries of
-  // a group without a count test.  wide: a row's offsets may not fit 12
-  // bits, so the list is marked as overflowed.
+  // self_off = i - that row's first candidate slot), which every replaying
+  // pass drops (r = 0), so a replay decodes all entries of a group without
+  // a count test.  wide: a row has more candidates than an entry can
+  // address, so the list is marked as overflowed.
   __device__ __forceinline__ void flush(uint4 *__restrict__ nl,
                                        int *__restrict__ ncnt, int stride,
                                        int i, int self_off, bool wide) {
-    const bool over = wide || ((c & 7) && (unsigned)self_off >= 4096u);
+    const int part = c - g * kPerGroup;  // entries of the partial group
+    const bool over =
+        wide || (part > 0 && (unsigned)self_off >= (1u << kOffBits));
     if (over || c > kMaxHits) {
       ncnt[i] = kMaxHits + 1;
       return;
     }
     ncnt[i] = c;
-    if (c & 7) {
-      const unsigned pad = (4u << 12) | (unsigned)self_off;
-      for (int k = c & 7; k < 8; ++k) push(pad);
-      nl[(size_t)(c >> 3) * stride + i] = make_uint4(w0, w1, w2, w3);
+    if (part > 0) {
+      const unsigned pad = (4u << kOffBits) | (unsigned)self_off;
+      for (int k = part; k < kPerGroup; ++k) push(pad);
+      nl[(size_t)g * stride + i] = make_uint4(w0, w1, w2, w3);
     }
   }
 };
 
-// Fine x cell at which the list entries of a particle in fine cell cx start
-// counting, in every row.
-__device__ __forceinline__ int list_window(const Scene &sc, int cx) {
-  return min(max(cx - kListWindow, 0), sc.nc[0] - 1);
-}
-
 // Stencil walk of a PAPER searching pass over the rows of the 9 that lie in
-// the grid: on_row(row, lb, b, e) once per row (lb its list base, [b, e)
-// its candidate slots), then f(j, X[j], (row << 12) - lb) for every
-// candidate, whose list entry is that value + j; candidate positions are
-// loaded four at a time so that four loads are in flight per wait.
+// the grid: on_row(row, b, e) once per row ([b, e) its candidate slots, b
+// the base of its list entries), then f(j, X[j], (row << kOffBits) - b)
+// for every candidate, whose list entry is that value + j; candidate
+// positions are loaded four at a time so that four loads are in flight per
+// wait.
 template <typename R, typename F>
 __device__ __forceinline__ void walk_rows(const Scene &sc,
                                           const int *__restrict__ cs,
@@ -654,7 +656,6 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
                                           R &&on_row, F &&f) {
   int cx, cy, cz;
   cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
-  const int win = list_window(sc, cx);
   for (int dz = 0; dz < 3; ++dz) {
     const int zz = cz - 1 + dz;
     if (zz < 0 || zz >= sc.nc[2]) continue;
@@ -662,12 +663,10 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
       const int yy = cy - 1 + dy;
       if (yy < 0 || yy >= sc.nc[1]) continue;
       const int row = dz * 3 + dy;
-      const int lb =
-          cs[sc.cell_base + (zz * sc.nc[1] + yy) * sc.nc[0] + win] + sc.shift;
       int b, e;
       row_range(sc, cs, xi, yy, zz, b, e);
-      on_row(row, lb, b, e);
-      const int re = (row << 12) - lb;
+      on_row(row, b, e);
+      const int re = (row << kOffBits) - b;
       int j = b;
       for (; j + 4 <= e; j += 4) {
         const float4 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2], a3 = X[j + 3];
@@ -681,11 +680,22 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
   }
 }
 
-// f(j, row, X[j]) for every j of N_i in recorded order, eight entries
-// resolved and their eight loads issued before any is used; the padding of
-// the last group names i itself, which every pass's 0 < r^2 test (or
-// j != i) drops.  rbase holds the 9 row bases the search stored; rb is LDS
-// scratch of 9 x blockDim.x ints.
+// Entry q of a list group (bits kGroupSkip + q kEntryBits up; three of the
+// eleven straddle two words).  q is a constant after unrolling.
+__device__ __forceinline__ unsigned group_entry(const unsigned (&ws)[4],
+                                                int q) {
+  const int p = kGroupSkip + q * kEntryBits;
+  const int wi = p >> 5, sh = p & 31;
+  if (sh + kEntryBits <= 32) return ws[wi] >> sh;
+  return __builtin_amdgcn_alignbit(ws[wi + 1], ws[wi], sh);
+}
+
+// f(j, row, X[j]) for every j of N_i in recorded order, the entries of a
+// group resolved and all their loads issued before any is used (more loads
+// in flight per wave measured faster than fewer VGPRs and more waves); the
+// padding of the last group names i itself, which every pass's 0 < r^2
+// test (or j != i) drops.  rbase holds the 9 row starts the search stored;
+// rb is LDS scratch of 9 x blockDim.x ints.
 template <typename T, typename F>
 __device__ __forceinline__ void for_neighbours(
     const Scene &sc, const int *__restrict__ cs, const Gather<float4> &XS,
@@ -695,35 +705,35 @@ __device__ __forceinline__ void for_neighbours(
   constexpr int kShift = sizeof(T) == 32 ? 5 : sizeof(T) == 16 ? 4 : 3;
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
-    // rb[row * bs + lane] = the row's list base (lanes of one row in
-    // consecutive banks), from the bases the search stored
+    // rb[row * bs + lane] = the row's start (lanes of one row in
+    // consecutive banks), from the starts the search stored
     const unsigned bs = blockDim.x;
     int *my = rb + threadIdx.x;
 #pragma unroll
     for (int r = 0; r < 9; ++r) my[r * bs] = rbase[(size_t)r * stride + i];
     // table address row * 4 bs + 4 lane (one 24-bit multiply-add); record
-    // byte offset (base + entry) << shift (one add-shift)
+    // byte offset (start + entry offset) << shift (one add-shift)
     const unsigned bs4 = bs * 4u, lane4 = threadIdx.x * 4u;
     const char *rbb = reinterpret_cast<const char *>(rb);
-    for (int g = 0; g * 8 < cnt; ++g) {
+    constexpr unsigned kOffMask = (1u << kOffBits) - 1u;
+    for (int g = 0; g * kPerGroup < cnt; ++g) {
       const uint4 w = nl[(size_t)g * stride + i];
       const unsigned ws[4] = {w.x, w.y, w.z, w.w};
-      int off[8], rows[8];
+      int off[kPerGroup], rows[kPerGroup];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const unsigned word = ws[q >> 1];
-        const unsigned row = (q & 1) ? word >> 28 : (word >> 12) & 0xfu;
-        const unsigned ent = (q & 1) ? (word >> 16) & 0xfffu : word & 0xfffu;
+      for (int q = 0; q < kPerGroup; ++q) {
+        const unsigned e = group_entry(ws, q);
+        const unsigned row = (e >> kOffBits) & 0xfu;
         rows[q] = (int)row;
         off[q] = (*reinterpret_cast<const int *>(rbb + row * bs4 + lane4) +
-                  (int)ent)
+                  (int)(e & kOffMask))
                  << kShift;
       }
-      T xv[8];
+      T xv[kPerGroup];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) xv[q] = X.at_bytes(off[q]);
+      for (int q = 0; q < kPerGroup; ++q) xv[q] = X.at_bytes(off[q]);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) f(off[q] >> kShift, rows[q], xv[q]);
+      for (int q = 0; q < kPerGroup; ++q) f(off[q] >> kShift, rows[q], xv[q]);
     }
   } else {
     const float h2 = sc.h2;
@@ -731,7 +741,7 @@ __device__ __forceinline__ void for_neighbours(
     const float3 xs = make_float3(s4.x, s4.y, s4.z);
     int row = 0;
     walk_rows(
-        sc, cs, XS, xs, [&](int r, int, int, int) { row = r; },
+        sc, cs, XS, xs, [&](int r, int, int) { row = r; },
         [&](int j, float4 sj, int) {
           const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
           const float r2 = rx * rx + ry * ry + rz * rz;
@@ -903,23 +913,23 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
   if (i - sc.slot_off >= sc.n) return;
   const float4 x4 = XS[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  HitRecorder rec{0u, 0u, 0u, 0u, 0};
+  HitRecorder rec{0u, 0u, 0u, 0u, 0, 0, kPerGroup - 1};
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
   const Gather<float4> xsg(XS, stride);
-  // the list base of every stencil row in the grid, for the replaying
-  // passes (row 4, the particle's own, also for the padding); rows outside
-  // the grid name no entry and get 0
+  // the first candidate slot of every stencil row in the grid, for the
+  // replaying passes (row 4, the particle's own, also for the padding);
+  // rows outside the grid name no entry and get 0
   int self_off = -1;
   unsigned in_grid = 0;
   bool wide = false;
   walk_rows(
       sc, cs, xsg, xi,
-      [&](int row, int lb, int b, int e) {
-        rbase[(size_t)row * stride + i] = lb;
+      [&](int row, int b, int e) {
+        rbase[(size_t)row * stride + i] = b;
         in_grid |= 1u << row;
-        if (row == 4) self_off = i - lb;
-        wide = wide || e - lb > 4096;
+        if (row == 4) self_off = i - b;
+        wide = wide || e - b > (1 << kOffBits);
       },
       [&](int j, float4 xj, int re) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;

@@ -21,17 +21,20 @@ constexpr int kXSub = 4;
 // row cut (the search runs at the positions the bins were built from).
 constexpr float kReachMargin = 1e-3f;
 
-// Neighbour-list entries are offsets from the first slot of fine x cell
-// max(fx - kListWindow, 0) of their row, fx the particle's own fine x cell:
-// the reach spans at most kXSub (1 + kReachMargin) fine cells, so every
-// visited slot lies at or after that cell, and a replaying pass finds the
-// base from fx alone.
-constexpr int kListWindow = kXSub + 1;
-static_assert(kReachMargin * kXSub < 1.0f, "reach must stay within the window");
+// Neighbour-list entries: the stencil row (0 .. 8, 4 bits) above kOffBits
+// bits of offset from the row's first candidate slot (the first slot of the
+// fine x cells the search visits in that row), kPerGroup entries packed in
+// one 16-byte group from bit kGroupSkip up.  A row whose candidate range is
+// longer than 2^kOffBits slots cannot be recorded (its particle walks its
+// rows in every pass instead).
+constexpr int kOffBits = 7;
+constexpr int kEntryBits = 4 + kOffBits;
+constexpr int kPerGroup = 128 / kEntryBits;
+constexpr int kGroupSkip = 128 - kPerGroup * kEntryBits;
 
 // Capacity of a particle's neighbour list (pairs within h at x*).
-constexpr int kMaxHits = 96;
-static_assert(kMaxHits % 8 == 0, "lists are stored in groups of 8 entries");
+constexpr int kMaxGroups = 9;
+constexpr int kMaxHits = kMaxGroups * kPerGroup;
 
 // Per-scene constants, resident in device memory and read by the kernels
 // with scalar loads (the scene index is uniform per work-group).  REF
@@ -96,10 +99,11 @@ struct DevBuffers {
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
   uint2 *xh;      // mixed precision: FP16 x - bin origin (sorted)
   int *kc;        // mixed precision: cell id of each sorted slot
-  uint4 *nl;      // PAPER neighbour lists: kMaxHits/8 groups of 8 x 16-bit
-                  // entries per slot
+  uint4 *nl;      // PAPER neighbour lists: kMaxGroups groups of kPerGroup
+                  // entries per slot, group g of slot i at [g * P + i]
   int *ncnt;      // list length per slot (> kMaxHits: list overflowed)
-  int *rbase;     // PAPER: list base slot of each stencil row, [row * P + i]
+  int *rbase;     // PAPER: first candidate slot of each stencil row (the
+                  // base of its list entries), [row * P + i]
   int *nbr;       // probe: neighbour counts (sorted)
   double *red;    // diagnostics partials
   int *err;       // device error word
