@@ -204,19 +204,23 @@ def test_paper_sphere_obstacle():
     assert d0.min() < r + 0.1 + 0.02   # the obstacle is in play
 
 
-def test_paper_list_overflow_matches_oracle():
+@pytest.mark.parametrize("spacing, dims", [(0.03, (10, 10, 10)),
+                                           (0.02, (12, 12, 12))])
+def test_paper_list_overflow_matches_oracle(spacing, dims):
     """A block at spacing 0.3 h has ~150 neighbours per particle, beyond the
-    99-entry list: the passes walk the rows again with the search's test at
-    x*, which must give the same neighbour sets as the oracle's lists."""
-    pos = scenarios.lattice((10, 10, 10), (0.3, 0.3, 0.3), seed=31,
-                            spacing=0.03, jitter=0.001)
+    99-entry list; at 0.2 h a stencil row also holds ~200 candidate slots,
+    beyond the 128 an entry can address.  The passes then walk the rows
+    again with the search's test at x*, which must give the same neighbour
+    sets as the oracle's lists."""
+    pos = scenarios.lattice(dims, (0.3, 0.3, 0.3), seed=31,
+                            spacing=spacing, jitter=0.001)
     parts = scenarios.particles_from_positions(pos)
     rho0 = float(np.median(orc.density_f64(
         scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_PAPER),
         np.c_[pos, np.zeros(len(pos))].astype(np.float64))))
     p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_PAPER,
                               rest_density=rho0)
-    say("paper list overflow", len(pos), "rho0", rho0)
+    say("paper list overflow", spacing, len(pos), "rho0", rho0)
     with pbf.Simulation(p, parts) as s:
         g = s.probe()
         assert g["nbr_count"].max() > 99     # lists overflow
