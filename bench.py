@@ -48,11 +48,17 @@ def algorithmic_bytes(cls: str, iterations: int, physics: int,
             "predict_count": 56.0,      # x, v in; x*, key, rank out
             "scatter": 24.0,
             "reorder": 88.0,
-            "search": 40.0 + lb,        # x* in; (x*, lambda), rho, count, list out
-            "lambda": 56.0 + lb,        # x*, x, count, list in; (x, lambda), rho out
-            "delta": 52.0 + lb + last_extra,  # x*, (x, lambda), count, list in; x out
-            "xsph": 116.0 + lb,         # x*, x, v, count, list in; v', omega, (x, |w|) out
-            "vorticity": 128.0 + lb,    # x*, (x,|w|), omega, v', count, list, id in; x, v, id out
+            # own record in; (x*, lambda), rho, count, 9 row starts, list out
+            "search": 76.0 + lb,
+            # x, count, 9 row starts, list in; (x, lambda), rho out
+            "lambda": 76.0 + lb,
+            # (x, lambda), count, 9 row starts, list in; x out (the last
+            # pass also reads rho and x_n and writes the 32-byte x, v record)
+            "delta": 72.0 + lb + last_extra,
+            # own (x, rho, v, m), count, row starts, list in; v', omega, (x, |w|) out
+            "xsph": 120.0 + lb,
+            # (x, |w|), omega, v', count, row starts, list, id in; x, v, id out
+            "vorticity": 128.0 + lb,
         }
     else:
         table = {
