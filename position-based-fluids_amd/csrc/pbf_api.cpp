@@ -336,6 +336,8 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   for (uint32_t s = 0; s < nscenes; ++s)
     vort = vort || params[s].vorticity_eps != 0.0f;
   sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
+  sh.scorr_n4 = 1;
+  for (const pbf::Scene &sc : S.scenes) sh.scorr_n4 &= sc.scorr_n == 4;
   for (int c = 0; c < pbf::kNumClasses; ++c)
     sh.block[c] = p0.block_size ? p0.block_size : 128;
   S.tune_pending = p0.block_size == 0;

@@ -31,7 +31,6 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
-#include <type_traits>
 
 namespace pbf {
 
@@ -1019,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_lambda_list_h(
 // XV = ((x, rho), (v, m)) instead, v = (x - x_n)/dt, for the XSPH pass.  WRITE_H (mixed
 // precision) also writes the FP16 bin-relative copy for the next FP16 lambda
 // pass.
-template <bool LAST, bool WRITE_H>
+template <bool LAST, bool WRITE_H, bool N4>
 __global__ __launch_bounds__(256) void k_delta_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ B,
@@ -1042,36 +1041,31 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   const int n = sc.scorr_n;
   const Gather<float4> xsg(XS, stride), bg(B, stride);
   // (lambda_i + lambda_j + s_corr) (h-r)^2 r/|r| summed branch-free (the
-  // out-of-reach entries selected to 0); the loop is compiled twice, for
-  // the paper's exponent n = 4 and for any n (uniform per scene)
-  const auto sum = [&](auto n4) {
-    for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
-                   [&](int, int, float4 xj) {
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float r2 = rx * rx + ry * ry + rz * rz;
-      const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
-      const float t = h2 - r2;
-      const float q = t * t * t * cq;  // W / W(dq)
-      float qn;
-      if constexpr (decltype(n4)::value) {
-        const float q2 = q * q;
-        qn = q2 * q2;
-      } else {
-        qn = 1.0f;
-        for (int e = 0; e < n; ++e) qn *= q;
-      }
-      const float coef = li + xj.w - kk * qn;
-      const float gs = in_reach(r2, h2) ? coef * hr * hr * inv : 0.0f;
-      ax += rx * gs;
-      ay += ry * gs;
-      az += rz * gs;
-    });
-  };
-  if (n == 4)
-    sum(std::true_type{});
-  else
-    sum(std::false_type{});
+  // out-of-reach entries selected to 0).  N4: the paper's exponent n = 4 in
+  // every scene (the host picks the instance; the loop over any n costs
+  // registers and occupancy)
+  for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
+                 [&](int, int, float4 xj) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float hr = h - r2 * inv;
+    const float t = h2 - r2;
+    const float q = t * t * t * cq;  // W / W(dq)
+    float qn;
+    if constexpr (N4) {
+      const float q2 = q * q;
+      qn = q2 * q2;
+    } else {
+      qn = 1.0f;
+      for (int e = 0; e < n; ++e) qn *= q;
+    }
+    const float coef = li + xj.w - kk * qn;
+    const float gs = in_reach(r2, h2) ? coef * hr * hr * inv : 0.0f;
+    ax += rx * gs;
+    ay += ry * gs;
+    az += rz * gs;
+  });
   // grad_i W = -spiky (h-r)^2 r/|r|
   const float c = -sc.spiky * sc.inv_rho0;
   const float3 dp = make_float3(ax * c, ay * c, az * c);
@@ -1442,6 +1436,20 @@ static void step_ref(const DevBuffers &d, Launcher &L) {
              d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err);
 }
 
+// A delta pass, at the s_corr exponent instance of the step's scenes.
+template <bool LAST, bool WRITE_H>
+static void delta_pass(const DevBuffers &d, Launcher &L) {
+  const int P = L.sh.slots;
+  if (L.sh.scorr_n4)
+    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_H, true>), 9,
+               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+               d.xold, d.nl, d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
+  else
+    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_H, false>), 9,
+               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+               d.xold, d.nl, d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
+}
+
 static void step_paper(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
   sort_stage<kPaper>(d, L, 1);
@@ -1461,19 +1469,12 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
       PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
                  XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
     }
-    if (it == sh.iterations - 1) {
-      PBF_GATHER(L, kClsDelta, (k_delta_paper<true, false>), 9, d.scenes,
-                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
-    } else if (fp16(it + 1)) {
-      PBF_GATHER(L, kClsDelta, (k_delta_paper<false, true>), 9, d.scenes,
-                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
-    } else {
-      PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
-                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
-    }
+    if (it == sh.iterations - 1)
+      delta_pass<true, false>(d, L);
+    else if (fp16(it + 1))
+      delta_pass<false, true>(d, L);
+    else
+      delta_pass<false, false>(d, L);
   }
   if (sh.vorticity) {
     PBF_GATHER(L, kClsXsph, k_xsph_omega, 9, d.scenes, d.block_scene, XS,
@@ -1531,9 +1532,7 @@ bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
       PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
                  XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
     } else {
-      PBF_GATHER(L, kClsDelta, (k_delta_paper<false, false>), 9, d.scenes,
-                 d.block_scene, XS, d.xb, d.cell_start, d.rho, d.xold, d.nl,
-                 d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
+      delta_pass<false, false>(d, L);
     }
   }
   return true;

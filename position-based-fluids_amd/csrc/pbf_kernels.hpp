@@ -40,6 +40,7 @@ struct StepShape {
   int iterations;
   int fp16_iterations;  // leading iterations gathering FP16 positions
   int vorticity;    // PAPER: vorticity confinement pass enabled
+  int scorr_n4;     // every scene has the s_corr exponent n = 4
   int block[kNumClasses];  // work-group size (64, 128 or 256) per class
 };
 

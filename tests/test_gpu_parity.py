@@ -170,6 +170,21 @@ def test_paper_vs_fp64_oracle():
     assert np.abs(gv[:, :3] - ov[:, :3]).max() <= 1e-3 * np.abs(ov[:, :3]).max()
 
 
+def test_paper_scorr_exponent_two_vs_fp64_oracle():
+    """s_corr with n = 2 runs the delta instance for any exponent (n = 4 has
+    its own); the two exponents move this block apart by ~5 cm in 10 steps,
+    so the bar below tells them apart."""
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=22, scorr_n=2.0)
+    p, parts = sc.params[0], sc.particles[0]
+    say("paper scorr n=2 vs fp64", sc.n)
+    gp, gv = run_gpu(p, parts, 10)
+    pos, vel = orc.state_from_particles(parts, np.float64)
+    op, ov = orc.run_f64(p, pos, vel, 10)
+    assert np.abs(gp[:, :3] - op[:, :3]).max() <= 1e-4
+    assert np.abs(gv[:, :3] - ov[:, :3]).max() <= 1e-3 * np.abs(ov[:, :3]).max()
+
+
 def test_paper_c0_vs_fp64_oracle():
     """The full 32,768-particle c0 dam in PAPER physics, 10 steps.  The FP32
     oracle itself sits 4.7e-5 m / 2.0e-3 m/s from the FP64 one here, so the
