@@ -409,7 +409,8 @@ def main():
     step_gbs = b_step * args.steps / (ms / 1e3) / 1e9
 
     out = {
-        "metric": "particle-steps/s",
+        "metric": ("particle-steps/s @1M (4 iters) & 256-scene batch vs plain, "
+                   "equal density err"),
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
