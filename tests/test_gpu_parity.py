@@ -541,6 +541,67 @@ def test_full_size_c1_properties():
     assert abs(d["momentum"][1]) > 0
 
 
+def test_full_size_c2_properties():
+    """8,388,608 particles in two columns of a mostly empty 38.4 m box
+    (56.6M fine cells): the sort and the cell table stay exact at full size
+    and two runs give identical bits."""
+    c2 = scenarios.config(2)
+    say("c2 full size", c2.n)
+    p, parts = c2.params[0], c2.particles[0]
+    with pbf.Simulation(p, parts) as s:
+        s.step(3)
+        pos, vel = s.dump()
+        g = s.probe()
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    ids = g["sorted_ids"]
+    assert np.array_equal(np.sort(ids), np.arange(c2.n))
+    assert np.all(np.diff(g["cell_of"][ids]) >= 0)
+    cs = g["cell_start"]
+    assert cs[0] == 0 and cs[-1] == c2.n and np.all(np.diff(cs) >= 0)
+    with pbf.Simulation(p, parts) as s2:
+        s2.step(3)
+        pos2, vel2 = s2.dump()
+    assert np.array_equal(pos, pos2) and np.array_equal(vel, vel2)
+
+
+def test_full_size_c3_batch_sampled_scenes():
+    """The 256-scene batch at full size: three scenes (the first, one in the
+    middle, the last) bit-identical to their single-scene runs."""
+    c3 = scenarios.config(3)
+    say("c3 full size", len(c3.params), "scenes", c3.n)
+    with pbf.BatchSimulation(c3.params, c3.particles) as b:
+        assert b.num_scenes() == len(c3.params)
+        b.step(3)
+        got = {k: b.dump_scene(k) for k in (0, 128, len(c3.params) - 1)}
+    for k, (bp, bv) in got.items():
+        assert np.isfinite(bp).all() and np.isfinite(bv).all()
+        gp, gv = run_gpu(c3.params[k], c3.particles[k], 3)
+        assert np.array_equal(bp, gp) and np.array_equal(bv, gv), k
+
+
+def test_full_size_c4_mixed_and_obstacle():
+    """4,194,304 particles around the seeded sphere, mixed precision: no
+    particle inside the obstacle's r + h shell, and the FP16 gathers stay
+    close to FP32 over 5 steps."""
+    c4 = scenarios.config(4)
+    p, parts = c4.params[0], c4.particles[0]
+    say("c4 full size", c4.n)
+    pm = p.copy()
+    pm.precision = pbf.PRECISION_MIXED
+    with pbf.Simulation(pm, parts) as s:
+        s.step(5)
+        mp, mv = s.dump()
+    with pbf.Simulation(p, parts) as s:
+        s.step(5)
+        fp, fv = s.dump()
+    assert np.isfinite(mp).all() and np.isfinite(mv).all()
+    centre, r = np.array(p.sphere[:3], np.float32), float(p.sphere[3])
+    assert r > 0
+    d = np.linalg.norm(mp[:, :3] - centre, axis=1)
+    assert d.min() >= r + 0.1 - 1e-4
+    assert np.abs(mp[:, :3] - fp[:, :3]).max() <= 1e-3
+
+
 def test_profile_and_timing_hooks():
     sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
                              physics=pbf.PHYSICS_PAPER)
