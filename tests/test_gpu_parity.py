@@ -5,7 +5,7 @@ Bars:
     neighbour counts): bit-exact;
   * REF physics trajectories vs the FP32 oracle (the reference kernels'
     arithmetic and summation order): bit-exact positions and velocities
-    after 1 and 10 steps (the north-star bar of 1e-5 relative is implied);
+    after 1 and 10 steps (a 1e-5 relative bar is implied);
   * PAPER physics vs the FP64 oracle: positions within 1e-4 m (1e-3 h),
     velocities within 1e-3 of the largest speed, after 10 steps;
   * batched scenes: bit-identical to the same scenes run one by one;
