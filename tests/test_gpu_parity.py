@@ -455,6 +455,25 @@ def test_batch_matches_single_scenes():
         assert np.array_equal(got[k][1], gv), k
 
 
+def test_batch_with_empty_scenes():
+    """Scenes without particles take no slots; the others are unaffected."""
+    a = scenarios.small_dam(dims=(6, 6, 6), box=(1.0, 1.0, 1.0),
+                            physics=pbf.PHYSICS_PAPER, seed=91)
+    b = scenarios.small_dam(dims=(9, 7, 5), box=(1.5, 1.0, 1.0),
+                            physics=pbf.PHYSICS_PAPER, seed=92)
+    empty = np.zeros((0, 10), np.float32)
+    params = [a.params[0], a.params[0], b.params[0], b.params[0]]
+    parts = [empty, a.particles[0], empty, b.particles[0]]
+    say("batch with empty scenes")
+    with pbf.BatchSimulation(params, parts) as s:
+        s.step(4)
+        got = [s.dump_scene(k) for k in range(4)]
+    assert got[0][0].shape == (0, 4) and got[2][0].shape == (0, 4)
+    for k in (1, 3):
+        gp, gv = run_gpu(params[k], parts[k], 4)
+        assert np.array_equal(got[k][0], gp) and np.array_equal(got[k][1], gv), k
+
+
 WAVES = [0.0, 0.1, 0.25, 0.3, 0.15, 0.05]
 
 
