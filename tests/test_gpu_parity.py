@@ -455,6 +455,30 @@ def test_batch_matches_single_scenes():
         assert np.array_equal(got[k][1], gv), k
 
 
+@pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
+def test_checkpoint_and_resume(physics):
+    """A dump is a checkpoint: a new handle created from the dumped state
+    (m = vel.w, x = pos.xyz, v = vel.xyz, input order) continues exactly as
+    the original, since every step re-sorts by (cell, id) from positions and
+    ids alone."""
+    sc = scenarios.small_dam(dims=(14, 14, 14), box=(2.0, 2.0, 2.0),
+                             physics=physics, seed=33)
+    p, parts = sc.params[0], sc.particles[0]
+    say("checkpoint/resume", physics)
+    with pbf.Simulation(p, parts) as s:
+        s.step(5)
+        pos5, vel5 = s.dump()
+        s.step(5)
+        pos10, vel10 = s.dump()
+    resumed = np.zeros_like(parts)
+    resumed[:, 0] = vel5[:, 3]
+    resumed[:, 1:4] = pos5[:, :3]
+    resumed[:, 4:7] = vel5[:, :3]
+    gp, gv = run_gpu(p, resumed, 5)
+    assert np.array_equal(gp[:, :3], pos10[:, :3])
+    assert np.array_equal(gv, vel10)
+
+
 def test_batch_with_empty_scenes():
     """Scenes without particles take no slots; the others are unaffected."""
     a = scenarios.small_dam(dims=(6, 6, 6), box=(1.0, 1.0, 1.0),
