@@ -213,10 +213,9 @@ def batch_vs_sequential(physics: int, steps: int, nscenes: int = 256):
     batch.step(2)
     batch.synchronize()
     ms_batch = batch.time_steps(steps)
-    batch.close()
     sims = [pbf.Simulation(p, parts) for p, parts in zip(sc.params, sc.particles)]
     for s in sims:
-        s.step(1)
+        s.step(2)
     for s in sims:
         s.synchronize()
     t0 = time.perf_counter()
@@ -225,6 +224,14 @@ def batch_vs_sequential(physics: int, steps: int, nscenes: int = 256):
     for s in sims:
         s.synchronize()
     ms_seq = (time.perf_counter() - t0) * 1e3
+    # both ran 2 + steps steps: the batched scenes must be the same bits
+    # (hence the same density error) as the scenes run one by one
+    same = True
+    for k in (0, nscenes // 2, nscenes - 1):
+        bp, bv = batch.dump_scene(k)
+        sp, sv = sims[k].dump()
+        same = same and np.array_equal(bp, sp) and np.array_equal(bv, sv)
+    batch.close()
     for s in sims:
         s.close()
     return {
@@ -236,6 +243,7 @@ def batch_vs_sequential(physics: int, steps: int, nscenes: int = 256):
         "speedup": ms_seq / ms_batch,
         "batched_ms_per_step": ms_batch / steps,
         "sequential_ms_per_step": ms_seq / steps,
+        "sampled_scenes_bit_identical": bool(same),
     }
 
 
