@@ -20,7 +20,9 @@
 //
 // Builtins the reference takes from its OpenCL implementation (sqrt,
 // length, division under -cl-fast-relaxed-math) are fixed here to IEEE
-// sqrt / division and length(v) = sqrt(x*x + y*y + z*z).
+// sqrt / division and length(v) = sqrt(x*x + y*y + z*z).  No reference test
+// or fixture pins what an OpenCL implementation returns for them: parity
+// unpinned at that boundary.
 //
 // Pins (tests/test_oracle.py): lattice closed forms for rho and lambda in
 // both modes, the two-particle scene of assets/scenarios/two.in worked by
