@@ -16,6 +16,7 @@ import sys
 
 import numpy as np
 import pytest
+from hypothesis import given, settings, strategies as st
 
 import pbf
 import pbf_oracle as orc
@@ -453,6 +454,75 @@ def test_batch_matches_single_scenes():
         gp, gv = run_gpu(c.params[k], c.particles[k], 6)
         assert np.array_equal(got[k][0], gp), k
         assert np.array_equal(got[k][1], gv), k
+
+
+def _random_scene(data, physics):
+    """A box of a random whole number of h-cells per axis with random
+    particles (some outside it) and random velocities."""
+    box = tuple(0.1 * data.draw(st.integers(lo, hi))
+                for lo, hi in ((5, 20), (5, 15), (5, 15)))
+    n = data.draw(st.integers(1, 1500))
+    seed = data.draw(st.integers(0, 2**31 - 1))
+    rng = np.random.default_rng(seed)
+    parts = np.zeros((n, 10), np.float32)
+    parts[:, 0] = 1.0
+    lo, hi = -0.05, 1.05
+    parts[:, 1:4] = (rng.uniform(lo, hi, (n, 3)) * np.array(box)).astype(np.float32)
+    parts[:, 4:7] = rng.normal(0.0, 0.5, (n, 3)).astype(np.float32)
+    return scenarios.make_params(box, physics), parts
+
+
+@settings(max_examples=10, deadline=None, derandomize=True)
+@given(data=st.data())
+def test_random_ref_scenes_match_oracle(data):
+    """Seeded property: REF on random boxes, counts and states stays
+    bit-identical to the FP32 oracle over 2 steps."""
+    p, parts = _random_scene(data, pbf.PHYSICS_REF)
+    say("random ref scene", parts.shape[0])
+    gp, gv = run_gpu(p, parts, 2)
+    pos, vel = orc.state_from_particles(parts)
+    op, ov = orc.run_f32(p, pos, vel, 2)
+    assert np.array_equal(gp, op) and np.array_equal(gv, ov)
+
+
+@settings(max_examples=6, deadline=None, derandomize=True)
+@given(data=st.data())
+def test_random_paper_batches_match_single_runs(data):
+    """Seeded property: a batch of 2-4 random PAPER scenes is bit-identical
+    to the scenes run one by one."""
+    k = data.draw(st.integers(2, 4))
+    scenes = [_random_scene(data, pbf.PHYSICS_PAPER) for _ in range(k)]
+    say("random paper batch", [s[1].shape[0] for s in scenes])
+    with pbf.BatchSimulation([s[0] for s in scenes], [s[1] for s in scenes]) as b:
+        b.step(2)
+        got = [b.dump_scene(q) for q in range(k)]
+    for q, (p, parts) in enumerate(scenes):
+        gp, gv = run_gpu(p, parts, 2)
+        assert np.array_equal(got[q][0], gp) and np.array_equal(got[q][1], gv), q
+
+
+@pytest.mark.parametrize("physics, precision", [
+    (pbf.PHYSICS_REF, pbf.PRECISION_FP32),
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_FP32),
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_MIXED)])
+def test_work_group_sizes_give_identical_bits(physics, precision):
+    """Each work-item computes one particle's sums in list (or stencil)
+    order, so the work-group size (64, 128, 256: what the autotuner picks
+    from) changes no bit."""
+    sc = scenarios.small_dam(dims=(18, 14, 12), box=(2.0, 2.0, 2.0),
+                             physics=physics, seed=41)
+    say("work-group sizes", physics, precision)
+    runs = []
+    for bs in (64, 128, 256):
+        p = sc.params[0].copy()
+        p.block_size = bs
+        p.precision = precision
+        with pbf.Simulation(p, sc.particles[0]) as s:
+            assert s.config()[0] == bs
+            s.step(4)
+            runs.append(s.dump())
+    for gp, gv in runs[1:]:
+        assert np.array_equal(gp, runs[0][0]) and np.array_equal(gv, runs[0][1])
 
 
 @pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
