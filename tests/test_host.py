@@ -7,6 +7,7 @@ import re
 
 import numpy as np
 import pytest
+from hypothesis import given, settings, strategies as st
 
 import pbf
 import scenarios
@@ -169,3 +170,28 @@ def test_splitmix64_known_values():
     v = scenarios.splitmix64(0, 3)
     assert [int(x) for x in v] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
                                    0x06C45D188009454F]
+
+
+@settings(max_examples=25, deadline=None, derandomize=True)
+@given(n=st.integers(0, 300), seed=st.integers(0, 2**31 - 1),
+       box=st.tuples(*[st.integers(1, 400)] * 3))
+def test_scenario_files_round_trip(tmp_path_factory, n, seed, box):
+    """Seeded property: particles and box parameters written in the
+    reference's .in / .par formats read back to the same float32 values."""
+    d = tmp_path_factory.mktemp("rt")
+    rng = np.random.default_rng(seed)
+    parts = np.zeros((n, 10), np.float32)
+    parts[:, :7] = rng.normal(0.0, 3.0, (n, 7)).astype(np.float32)
+    parts[:, 0] = np.abs(parts[:, 0]) + np.float32(0.5)   # masses > 0
+    scenarios.write_in(str(d / "p.in"), parts)
+    got = pbf.read_particles(str(d / "p.in"))
+    assert got.shape == (n, 10)
+    assert np.array_equal(got[:, :7], parts[:, :7])
+    p = scenarios.make_params(tuple(0.1 * b for b in box), pbf.PHYSICS_REF,
+                              rest_density=float(rng.uniform(100, 9000)))
+    scenarios.write_par(str(d / "s.par"), p, "p.in")
+    q, part_file = pbf.read_par(str(d / "s.par"))
+    assert part_file == "p.in"
+    for k in ("time_step_length", "time_end", "x_min", "x_max", "y_min",
+              "y_max", "z_min", "z_max", "x_n", "y_n", "z_n", "rest_density"):
+        assert np.float32(getattr(q, k)) == np.float32(getattr(p, k)), k
