@@ -195,3 +195,16 @@ def test_scenario_files_round_trip(tmp_path_factory, n, seed, box):
     for k in ("time_step_length", "time_end", "x_min", "x_max", "y_min",
               "y_max", "z_min", "z_max", "x_n", "y_n", "z_n", "rest_density"):
         assert np.float32(getattr(q, k)) == np.float32(getattr(p, k)), k
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it says so and fails
+    before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"),
+                        "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr and r.stdout == ""
