@@ -220,9 +220,11 @@ def test_paper_sphere_obstacle():
     assert d0.min() < r + 0.1 + 0.02   # the obstacle is in play
 
 
-@pytest.mark.parametrize("spacing, dims", [(0.03, (10, 10, 10)),
-                                           (0.02, (12, 12, 12))])
-def test_paper_list_overflow_matches_oracle(spacing, dims):
+@pytest.mark.parametrize("spacing, dims, precision", [
+    (0.03, (10, 10, 10), pbf.PRECISION_FP32),
+    (0.02, (12, 12, 12), pbf.PRECISION_FP32),
+    (0.03, (10, 10, 10), pbf.PRECISION_MIXED)])
+def test_paper_list_overflow_matches_oracle(spacing, dims, precision):
     """A block at spacing 0.3 h has ~150 neighbours per particle, beyond the
     99-entry list; at 0.2 h a stencil row also holds ~200 candidate slots,
     beyond the 128 an entry can address.  The passes then walk the rows
@@ -235,8 +237,8 @@ def test_paper_list_overflow_matches_oracle(spacing, dims):
         scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_PAPER),
         np.c_[pos, np.zeros(len(pos))].astype(np.float64))))
     p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_PAPER,
-                              rest_density=rho0)
-    say("paper list overflow", spacing, len(pos), "rho0", rho0)
+                              rest_density=rho0, precision=precision)
+    say("paper list overflow", spacing, precision, len(pos), "rho0", rho0)
     with pbf.Simulation(p, parts) as s:
         g = s.probe()
         assert g["nbr_count"].max() > 99     # lists overflow
@@ -244,8 +246,14 @@ def test_paper_list_overflow_matches_oracle(spacing, dims):
         gp, gv = s.dump()
     ps, vs = orc.state_from_particles(parts, np.float64)
     op, ov = orc.run_f64(p, ps, vs, 2)
-    assert np.abs(gp[:, :3] - op[:, :3]).max() <= 1e-4
-    assert np.abs(gv[:, :3] - ov[:, :3]).max() <= 1e-3 * max(np.abs(ov[:, :3]).max(), 1.0)
+    vbar = 1e-3 * max(np.abs(ov[:, :3]).max(), 1.0)
+    if precision == pbf.PRECISION_FP32:
+        assert np.abs(gp[:, :3] - op[:, :3]).max() <= 1e-4
+        assert np.abs(gv[:, :3] - ov[:, :3]).max() <= vbar
+    else:  # the mixed bars of test_mixed_vs_fp64_oracle
+        assert np.abs(gp[:, :3] - op[:, :3]).max() <= MIXED_POS_TOL
+        assert np.abs(gv[:, :3] - ov[:, :3]).max() <= \
+            max(vbar, MIXED_POS_TOL / p.time_step_length)
 
 
 # ----------------------------------------------------------------------------
