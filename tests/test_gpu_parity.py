@@ -351,7 +351,8 @@ def test_cpp_runner_matches_c_abi(tmp_path, golden_dir):
     import subprocess
     shutil.copy(os.path.join(golden_dir, "dam.in"), tmp_path / "dam.in")
     text = open(os.path.join(golden_dir, "dam.par")).read()
-    text += "\npart_out_freq 5\npart_out_name_base out_\n"
+    text += ("\npart_out_freq 5\npart_out_name_base out_\n"
+             "vtk_out_freq 10\nvtk_out_name_base v_\n")
     (tmp_path / "dam.par").write_text(text)
     exe = os.path.join(os.path.dirname(pbf.LIB_PATH), "pbf_run")
     say("pbf_run dam.par --steps 10")
@@ -369,6 +370,15 @@ def test_cpp_runner_matches_c_abi(tmp_path, golden_dir):
             assert np.array_equal(got[:, 1:4], pos[:, :3]), k
             assert np.array_equal(got[:, 4:7], vel[:, :3]), k
             assert np.array_equal(got[:, 0], vel[:, 3]), k
+    # the VTK dump of step 10: POINTS then VECTORS velocity, 9 digits
+    lines = open(tmp_path / "v_1.vtk").read().split("\n")
+    n = len(parts)
+    assert lines[0].startswith("# vtk DataFile") and lines[4] == f"POINTS {n} float"
+    pts = np.array([l.split() for l in lines[5:5 + n]], np.float64).astype(np.float32)
+    assert lines[5 + n] == f"POINT_DATA {n}"
+    vec = np.array([l.split() for l in lines[7 + n:7 + 2 * n]],
+                   np.float64).astype(np.float32)
+    assert np.array_equal(pts, pos[:, :3]) and np.array_equal(vec, vel[:, :3])
 
 
 # ----------------------------------------------------------------------------
