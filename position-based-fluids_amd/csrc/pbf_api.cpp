@@ -80,8 +80,10 @@ void validate(const pbf_params &p) {
     invalid("cell counts x_n, y_n, z_n must be integers >= 1");
   if (!(p.x_max > p.x_min && p.y_max > p.y_min && p.z_max > p.z_min))
     invalid("box must have max > min on every axis");
-  if ((double)p.x_n * p.y_n * p.z_n > 1.0e9)
-    invalid("too many cells (limit 1e9)");
+  // PAPER bins x kXSub times finer; cell ids are 32-bit
+  const double xsub = p.physics == PBF_PHYSICS_PAPER ? pbf::kXSub : 1;
+  if ((double)p.x_n * xsub * p.y_n * p.z_n > (double)(1 << 30))
+    invalid("too many cells (limit 2^30, PAPER x cells counted 4 times)");
   if (!(p.time_step_length > 0.0f)) invalid("timestep_length must be > 0");
   if (!(p.rest_density > 0.0f)) invalid("restdensity must be > 0");
   if (p.physics != PBF_PHYSICS_REF && p.physics != PBF_PHYSICS_PAPER)

@@ -115,6 +115,16 @@ def test_create_rejects_bad_params():
     bad.x_max = bad.x_min
     with pytest.raises(pbf.PBFError):
         pbf.Simulation(bad, parts)
+    # 1000^3 cells fit 2^30 for REF, not once PAPER splits x in 4 (the
+    # product overflowed a 32-bit cell count before it was checked)
+    big = pbf.default_params(pbf.PHYSICS_PAPER)
+    for name in ("time_step_length", "rest_density"):
+        setattr(big, name, getattr(p, name))
+    big.x_max = big.y_max = big.z_max = 100.0
+    big.x_n = big.y_n = big.z_n = 1000.0
+    with pytest.raises(pbf.PBFError) as e:
+        pbf.Simulation(big, parts)
+    assert e.value.code == pbf.ERR_INVALID and "too many cells" in str(e.value)
     # mixed precision: PAPER physics only, 1 <= fp32_iterations <= iterations
     bad = p.copy()
     bad.precision, bad.fp32_iterations = pbf.PRECISION_MIXED, 1
