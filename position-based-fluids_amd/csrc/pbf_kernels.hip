@@ -754,11 +754,48 @@ __device__ __forceinline__ void for_neighbours(
 // x* alternating between two buffers)
 // ---------------------------------------------------------------------------
 
+// REF walk with the pair arithmetic taken out of the candidate loop.  The
+// walk (for_stencil_ref's order) only tests j != i, 0 < r^2 < h^2 and puts
+// the slots that pass in a per-lane queue in LDS (q: kRefQueue x blockDim.x
+// ints); whenever a lane's queue is full, every lane runs pair(j) over its
+// queued slots in order and empties it, and once more at the end.  A lane
+// thus sums over the same pairs in the same order as with for_stencil_ref,
+// while the wave runs the costly IEEE pair arithmetic about once per hit of
+// its busiest lane instead of once per candidate slot.  pair(j) repeats the
+// test (same operations, same result) inside the reference pair function.
+// Used by the lambda pass, whose pair term (two square roots, three
+// divisions) is the costliest: about 5 % off a c1 step.  The delta and XSPH
+// passes measured slower with it (their terms cost less than re-gathering the
+// queued neighbours), as did queues of 8 and 32 slots.
+constexpr int kRefQueue = 16;
+
+template <typename F>
+__device__ __forceinline__ void for_stencil_ref_queued(
+    const Scene &sc, const int *__restrict__ cs, const float4 *__restrict__ X,
+    float3 xi, int i, int *q, F &&pair) {
+  const unsigned bs = blockDim.x;
+  int *my = q + threadIdx.x;
+  int n = 0;
+  auto drain = [&] {
+    for (int k = 0; k < n; ++k) pair(my[k * bs]);
+    n = 0;
+  };
+  const float h2 = sc.h2;
+  for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+    float rx, ry, rz;
+    const float r2 = dist2_exact(xi, X[j], rx, ry, rz);
+    if (j != i && r2 > 0.0f && r2 < h2) my[n++ * bs] = j;
+    if (__ballot(n == kRefQueue)) drain();
+  });
+  drain();
+}
+
 // compute_scaling.cl:12-140.
 __global__ __launch_bounds__(256) void k_lambda_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
     float *__restrict__ lam, float *__restrict__ rho) {
+  extern __shared__ int rq_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
@@ -766,8 +803,8 @@ __global__ __launch_bounds__(256) void k_lambda_ref(
   const float4 x4 = X[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
-  for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-    if (j != i) ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
+  for_stencil_ref_queued(sc, cs, X, xi, i, rq_lds, [&](int j) {
+    ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
   });
   lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
   rho[i] = dens;
@@ -1015,9 +1052,9 @@ __global__ __launch_bounds__(256) void k_lambda_list_h(
 }
 
 // eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST writes
-// XV = ((x, rho), (v, m)) instead, v = (x - x_n)/dt, for the XSPH pass.  WRITE_H (mixed
-// precision) also writes the FP16 bin-relative copy for the next FP16 lambda
-// pass.
+// XV = ((x, rho), (v, m)) instead, v = (x - x_n)/dt, for the XSPH pass.
+// WRITE_H (mixed precision) also writes the FP16 bin-relative copy for the
+// next FP16 lambda pass.
 template <bool LAST, bool WRITE_H, bool N4>
 __global__ __launch_bounds__(256) void k_delta_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
@@ -1421,8 +1458,8 @@ static void step_ref(const DevBuffers &d, Launcher &L) {
   sort_stage<kRef>(d, L, 1);
   float4 *cur = d.xa, *nxt = d.xb;
   for (int it = 0; it < sh.iterations; ++it) {
-    PBF_GATHER(L, kClsLambda, k_lambda_ref, 0, d.scenes, d.block_scene, cur,
-               d.cell_start, d.lam, d.rho);
+    PBF_GATHER(L, kClsLambda, k_lambda_ref, kRefQueue, d.scenes,
+               d.block_scene, cur, d.cell_start, d.lam, d.rho);
     if (it == sh.iterations - 1) {
       PBF_GATHER(L, kClsDelta, k_delta_ref<true>, 0, d.scenes, d.block_scene,
                  cur, d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
@@ -1519,8 +1556,8 @@ bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
   for (int r = 0; r < reps; ++r) {
     if (sh.physics == kRef) {
       if (cls == kClsLambda)
-        PBF_GATHER(L, kClsLambda, k_lambda_ref, 0, d.scenes, d.block_scene,
-                   d.xa, d.cell_start, d.lam, d.rho);
+        PBF_GATHER(L, kClsLambda, k_lambda_ref, kRefQueue, d.scenes,
+                   d.block_scene, d.xa, d.cell_start, d.lam, d.rho);
       else
         PBF_GATHER(L, kClsDelta, k_delta_ref<false>, 0, d.scenes,
                    d.block_scene, d.xa, d.cell_start, d.lam, d.rho, d.xold,
@@ -1545,8 +1582,8 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
     sort_stage<kRef>(d, L, 1);
     k_count_nbrs<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                             d.cell_start, d.nbr);
-    k_lambda_ref<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
-                                      d.cell_start, d.lam, d.rho);
+    k_lambda_ref<<<grid, bs, kRefQueue * bs * sizeof(int), st>>>(
+        d.scenes, d.block_scene, d.xa, d.cell_start, d.lam, d.rho);
   } else {
     sort_stage<kPaper>(d, L, 1);
     k_count_nbrs<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
