@@ -851,7 +851,9 @@ __device__ __forceinline__ void write_final(int i, float3 xs, float4 v,
   pos[i] = make_float4(xs.x, xs.y, xs.z, len);
   vel[i] = v;
   pid[i] = pid_s[i];
-  if (!(xs.x == xs.x && xs.y == xs.y && xs.z == xs.z && len == len))
+  // len = |v| is non-finite if any velocity component is
+  if (!(__builtin_isfinite(xs.x) && __builtin_isfinite(xs.y) &&
+        __builtin_isfinite(xs.z) && __builtin_isfinite(len)))
     atomicOr(err, 1);
 }
 

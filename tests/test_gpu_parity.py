@@ -457,6 +457,38 @@ def test_nan_particle_is_reported(physics):
     assert e.value.code == pbf.ERR_NUMERIC
 
 
+def test_zero_density_neighbour_in_xsph():
+    """Two particles beyond h, one 0.02 m from the x-min wall, one iteration:
+    both have density 0 in the only lambda pass and the delta pass clamps the
+    wall particle within h of the other.  REF divides by that zero density
+    as the reference's XSPH sum does (apply_vorticity_and_viscosity.cl:61):
+    the velocities are infinite and the step reports PBF_ERR_NUMERIC, as the
+    FP32 oracle's velocities are non-finite.  PAPER skips the neighbour (and
+    its lists of the step do not hold the pair): finite, as the oracle."""
+    say("edge: zero-density XSPH neighbour")
+    parts = np.zeros((2, 10), np.float32)
+    parts[:, 0] = 1.0
+    parts[0, 1:4] = [0.02, 0.5, 0.5]
+    parts[1, 1:4] = [0.18, 0.5, 0.5]
+    for physics in (pbf.PHYSICS_REF, pbf.PHYSICS_PAPER):
+        p = scenarios.make_params((1.0, 1.0, 1.0), physics)
+        p.gravity_y, p.iterations = 0.0, 1
+        pos, vel = orc.state_from_particles(parts)
+        xo, vo = orc.run_f32(p, pos, vel, 1)
+        with pbf.Simulation(p, parts) as s:
+            s.step(1)
+            if physics == pbf.PHYSICS_REF:
+                assert not np.isfinite(vo[:, :3]).all()
+                with pytest.raises(pbf.PBFError) as e:
+                    s.synchronize()
+                assert e.value.code == pbf.ERR_NUMERIC
+            else:
+                assert np.isfinite(vo).all()
+                x, v = s.dump()
+                assert np.abs(x[:, :3] - xo[:, :3]).max() < 1e-6
+                assert np.abs(v[:, :3] - vo[:, :3]).max() < 1e-4
+
+
 # ----------------------------------------------------------------------------
 # batching, graphs, determinism, full size
 # ----------------------------------------------------------------------------

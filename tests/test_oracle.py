@@ -326,3 +326,35 @@ def test_literal_scorr_k01_diverges():
     p.scorr_k = 0.001
     _, v = orc.run_f64(p, pos, vel, 5)
     assert np.all(np.abs(v[:, :3]) < 5.0)
+
+
+def _zero_density_pair():
+    """Two particles 0.16 m apart (beyond h = 0.1), one 0.02 m from the x-min
+    wall; one iteration, no gravity.  In the only lambda pass both have no
+    neighbour (density 0); the delta pass clamps the wall particle to
+    x_min + h, within h of the other, before the XSPH pass."""
+    parts = np.zeros((2, 10), np.float32)
+    parts[:, 0] = 1.0
+    parts[0, 1:4] = [0.02, 0.5, 0.5]
+    parts[1, 1:4] = [0.18, 0.5, 0.5]
+    return parts
+
+
+def test_ref_xsph_divides_by_a_zero_density():
+    """REF keeps the reference's unguarded 1/rho_j of the XSPH sum
+    (apply_vorticity_and_viscosity.cl:61): a neighbour that had no pair in
+    the last lambda pass but is within h afterwards gives an infinite
+    velocity.  This is what ends a long REF run of the 1M-particle dam break
+    (first at step 82).  PAPER skips such a neighbour and stays finite."""
+    parts = _zero_density_pair()
+    p = params(gravity_y=0.0, iterations=1)
+    pos, vel = orc.state_from_particles(parts)
+    x, v = orc.run_f32(p, pos, vel, 1)
+    # clamped to x_min + h, through the update x* + (future - x*)
+    x0, fut = np.float32(0.02), np.float32(np.float32(0.0) + fmt6(0.1))
+    assert x[0, 0] == np.float32(x0 + np.float32(fut - x0))
+    assert np.isfinite(x[:, :3]).all()
+    assert not np.isfinite(v[:, 0]).any()
+    q = params(physics=pbf.PHYSICS_PAPER, gravity_y=0.0, iterations=1)
+    x, v = orc.run_f64(q, pos.astype(np.float64), vel.astype(np.float64), 1)
+    assert np.isfinite(x).all() and np.isfinite(v).all()
