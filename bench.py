@@ -384,11 +384,21 @@ def main():
 
     sim = pbf.Simulation(params, parts)
     cells = sim.num_cells()
-    sim.step(args.warmup)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    ms = sim.time_steps(args.steps)          # HIP events around the graph launches
-    sim.synchronize()
+    try:
+        sim.step(args.warmup)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        ms = sim.time_steps(args.steps)      # HIP events around the graph launches
+        sim.synchronize()
+    except pbf.PBFError as e:
+        if e.code != pbf.ERR_NUMERIC:
+            raise
+        # REF keeps the reference's unguarded 1/rho_j in XSPH
+        # (apply_vorticity_and_viscosity.cl:61); on the 1M-particle dam break
+        # it turns a velocity infinite at step 82
+        print(f"[bench] the state became non-finite within {args.warmup + args.steps} "
+              f"steps ({args.physics} physics): {e}", file=sys.stderr, flush=True)
+        sys.exit(3)
     wall = time.perf_counter() - t0
     value = n * args.steps / (ms / 1e3)
     print(f"[bench] timed {args.steps} steps: {ms:.3f} ms (wall {wall*1e3:.1f} ms)",
