@@ -575,6 +575,35 @@ def test_work_group_sizes_give_identical_bits(physics, precision):
         assert np.array_equal(gp, runs[0][0]) and np.array_equal(gv, runs[0][1])
 
 
+def test_autotuner_restores_state_and_writes_cache(tmp_path, monkeypatch):
+    """With block_size 0 and at least 2^18 slots the first step times every
+    kernel class at 64, 128 and 256 work-items on the handle's own state.
+    The state is restored afterwards: the tuned run equals a run at a fixed
+    size bit for bit (sizes change no bit).  The choice is appended to the
+    file PBF_TUNE_CACHE names, one line per problem shape."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("PBF_TUNE_CACHE", str(cache))
+    sc = scenarios.small_dam(dims=(64, 64, 64), box=(4.0, 4.0, 4.0),
+                             physics=pbf.PHYSICS_PAPER, seed=5)
+    assert sc.n >= 1 << 18
+    say("autotuner", sc.n)
+    runs = []
+    for bs in (0, 128):
+        p = sc.params[0].copy()
+        p.block_size = bs
+        with pbf.Simulation(p, sc.particles[0]) as s:
+            s.step(3)
+            runs.append(s.dump())
+            assert s.config()[0] in (64, 128, 256)
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    lines = cache.read_text().split("\n")
+    assert len([l for l in lines if l]) == 1
+    fields = lines[0].split()
+    assert fields[0].startswith("gfx")
+    assert len(fields) > 2 and all(f in ("64", "128", "256") for f in fields[1:])
+
+
 @pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
 def test_checkpoint_and_resume(physics):
     """A dump is a checkpoint: a new handle created from the dumped state
