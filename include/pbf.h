@@ -32,7 +32,7 @@ extern "C" {
 #define PBF_ERR_INVALID 1   /* bad argument (null, size, range)              */
 #define PBF_ERR_HIP 2       /* HIP runtime failure (no device, OOM, launch)  */
 #define PBF_ERR_STATE 3     /* call not valid for this handle                */
-#define PBF_ERR_NUMERIC 4   /* device error word set (NaN position)          */
+#define PBF_ERR_NUMERIC 4   /* a position or velocity became NaN or infinite */
 
 /* ---- physics modes ----------------------------------------------------- */
 /* PBF_PHYSICS_REF: the equations of the reference kernels
