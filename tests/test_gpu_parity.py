@@ -493,9 +493,19 @@ def test_zero_density_neighbour_in_xsph():
 # batching, graphs, determinism, full size
 # ----------------------------------------------------------------------------
 
-def test_batch_matches_single_scenes():
-    c = scenarios.batch_config(nscenes=5, nmin=300, nmax=3000, seed=77)
-    say("batch vs single", c.meta["counts"])
+@pytest.mark.parametrize("physics,precision", [
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_FP32),
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_MIXED),
+    (pbf.PHYSICS_REF, pbf.PRECISION_FP32)])
+def test_batch_matches_single_scenes(physics, precision):
+    """Scenes of uneven sizes and boxes advanced in one launch per kernel
+    give the bits of each scene run alone, in both physics modes and with
+    the FP16 lambda passes of the mixed precision."""
+    c = scenarios.batch_config(nscenes=5, physics=physics, nmin=300,
+                               nmax=3000, seed=77)
+    for p in c.params:
+        p.precision = precision
+    say("batch vs single", physics, precision, c.meta["counts"])
     with pbf.BatchSimulation(c.params, c.particles) as b:
         assert b.num_scenes() == 5
         b.step(6)
