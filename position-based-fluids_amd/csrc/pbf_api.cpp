@@ -231,6 +231,9 @@ struct pbf_sim {
   double *diag_out = nullptr;
   float wave = 0.0f;
   bool tune_pending = false;  // block size still to be chosen
+  // the predicted positions, keys, ranks and counts of the current state
+  // are in place (a step needs them; probes and the autotuner discard them)
+  bool predicted = false;
 
   template <typename T>
   T *alloc(size_t count) {
@@ -268,9 +271,18 @@ struct pbf_sim {
     PBF_HIP(hipMemcpyAsync(&e, d.err, sizeof(int), hipMemcpyDeviceToHost,
                            stream));
     PBF_HIP(hipStreamSynchronize(stream));
+    if (e & 2)
+      throw ApiError(PBF_ERR_STATE,
+                     "internal error: cell counts disagree with cell keys");
     if (e)
       throw ApiError(PBF_ERR_NUMERIC,
                      "non-finite particle position or velocity");
+  }
+
+  void ensure_predicted() {
+    if (predicted) return;
+    pbf::enqueue_predict(d, shape, stream);
+    predicted = true;
   }
 };
 
@@ -524,6 +536,7 @@ void autotune(pbf_sim &S) {
       copy(S.d.pos, spos, P * sizeof(float4));
       copy(S.d.vel, svel, P * sizeof(float4));
       copy(S.d.pid, spid, P * sizeof(int));
+      pbf::enqueue_predict(S.d, S.shape, S.stream);
       pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);  // warm-up
       EventTimer timer;
       for (int t = 0; t < 2; ++t)
@@ -544,10 +557,13 @@ void autotune(pbf_sim &S) {
     copy(S.d.pos, spos, P * sizeof(float4));
     copy(S.d.vel, svel, P * sizeof(float4));
     copy(S.d.pid, spid, P * sizeof(int));
+    S.predicted = false;  // the trial steps left their own
+    S.ensure_predicted();
     PBF_HIP(hipStreamSynchronize(S.stream));
     std::copy(best_b.begin(), best_b.end(), S.shape.block);
     tune_store(key, best_b);
   } catch (...) {
+    S.predicted = false;
     release();
     throw;
   }
@@ -650,6 +666,7 @@ int pbf_step(pbf_sim *S, uint32_t nsteps) {
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
     ensure_graph(*S);
+    S->ensure_predicted();
     for (uint32_t t = 0; t < nsteps; ++t)
       PBF_HIP(hipGraphLaunch(S->graph, S->stream));
   });
@@ -769,6 +786,7 @@ int pbf_probe(pbf_sim *S, int32_t *cell_of, int32_t *sorted_ids,
     }
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
+    S->predicted = false;
     pbf::enqueue_probe(S->d, S->shape, S->stream);
     const size_t P = (size_t)S->shape.slots, M = (size_t)S->shape.cells;
     Host h;
@@ -816,6 +834,7 @@ int pbf_diagnostics(pbf_sim *S, double out[8]) {
     if (S->shape.slots == 0) return;
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
+    S->predicted = false;
     pbf::enqueue_diag(S->d, S->shape, S->stream, S->diag_out);
     double h[8];
     PBF_HIP(hipMemcpyAsync(h, S->diag_out, sizeof h, hipMemcpyDeviceToHost,
@@ -849,6 +868,7 @@ int pbf_profile_steps(pbf_sim *S, uint32_t nsteps, double *ms,
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
     autotune(*S);
+    S->ensure_predicted();
     EventTimer timer;
     for (uint32_t t = 0; t < nsteps; ++t)
       pbf::enqueue_step(S->d, S->shape, S->stream, &timer);
@@ -872,6 +892,7 @@ int pbf_time_steps(pbf_sim *S, uint32_t nsteps, double *ms) {
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
     ensure_graph(*S);
+    S->ensure_predicted();
     hipEvent_t a, b;
     PBF_HIP(hipEventCreate(&a));
     PBF_HIP(hipEventCreate(&b));
@@ -902,6 +923,7 @@ int pbf_time_kernel(pbf_sim *S, const char *cls, uint32_t reps, double *ms) {
     PBF_HIP(hipSetDevice(S->device));
     S->upload_scenes();
     ensure_graph(*S);
+    S->ensure_predicted();
     PBF_HIP(hipGraphLaunch(S->graph, S->stream));  // fresh temporaries
     // the launches are captured into a graph, as in a step
     hipGraph_t g = nullptr;

@@ -407,23 +407,19 @@ __device__ __forceinline__ float4 mul_exact(float c, float x, float y,
 // neighbour search
 // ---------------------------------------------------------------------------
 
+// Slot i of the state (x, v): predict x* = x + dt (v + dt g) when
+// `predict` (predict_positions.cl:5-9), bin it (update_cells.cl:9-15) and
+// take an arrival rank in its cell.  Padding slots sit at the end of a
+// scene's range and a wave never spans two scenes, so the valid lanes of a
+// wave are a prefix of it; lanes past that prefix either pass valid = false
+// or have left the kernel (the ballots below see them as invalid).
 template <int PH>
-__global__ __launch_bounds__(256) void k_predict_count(
-    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+__device__ __forceinline__ void predict_count_slot(
+    const Scene &sc, int i, bool valid, float4 x, float4 v, int predict,
     float4 *__restrict__ pred_u, int *__restrict__ key,
-    int *__restrict__ rank, int *__restrict__ count, int predict) {
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  // Padding slots sit at the end of a scene's range and a wave never spans
-  // two scenes, so the valid lanes of a wave are a prefix of it.  They stay
-  // in the wave (no early return) for the cross-lane steps below.
-  const bool valid = i - sc.slot_off < sc.n;
+    int *__restrict__ rank, int *__restrict__ count) {
   int k = -1;
   if (valid) {
-    const float4 x = pos[i];
-    float4 v = vel[i];
     float3 p = make_float3(x.x, x.y, x.z);
     if (predict) p = predict_exact(sc, x, v);
     int cx, cy, cz;
@@ -452,6 +448,26 @@ __global__ __launch_bounds__(256) void k_predict_count(
   if (head) base = atomicAdd(&count[k], next - lane);
   base = __shfl(base, valid ? first : 0, 64);
   if (valid) rank[i] = base + lane - first;
+}
+
+template <int PH>
+__global__ __launch_bounds__(256) void k_predict_count(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+    float4 *__restrict__ pred_u, int *__restrict__ key,
+    int *__restrict__ rank, int *__restrict__ count, int predict) {
+  const int lb = logical_block();
+  const Scene &sc = scenes[scene_of(block_scene, lb)];
+  const int i = lb * blockDim.x + threadIdx.x;
+  // every lane stays for the cross-lane steps of predict_count_slot
+  const bool valid = i - sc.slot_off < sc.n;
+  float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v = x;
+  if (valid) {
+    x = pos[i];
+    v = vel[i];
+  }
+  predict_count_slot<PH>(sc, i, valid, x, v, predict, pred_u, key, rank,
+                         count);
 }
 
 // Exclusive scan over a 256-thread block; *total receives the block sum.
@@ -532,12 +548,18 @@ __global__ __launch_bounds__(256) void k_scatter(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const int *__restrict__ key, const int *__restrict__ rank,
     const int *__restrict__ pid, const int *__restrict__ cell_start,
-    int *__restrict__ perm, int *__restrict__ ppid) {
+    int *__restrict__ perm, int *__restrict__ ppid, int *__restrict__ err) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
   const int slot = cell_start[key[i]] + rank[i] + sc.shift;
+  // counts that disagree with the keys would place a slot outside its
+  // scene: reported (error bit 2), never written
+  if ((unsigned)(slot - sc.slot_off) >= (unsigned)sc.n) {
+    atomicOr(err, 2);
+    return;
+  }
   perm[slot] = i;
   ppid[slot] = pid[i];
 }
@@ -558,7 +580,10 @@ __global__ __launch_bounds__(256) void k_reorder(
   const int src = perm[t];
   const int p = ppid[t];
   const int c = key[src];
-  const int lo = cell_start[c] + sc.shift, hi = cell_start[c + 1] + sc.shift;
+  // clamped to the scene (only counts that disagree with the keys, which
+  // k_scatter reports, reach past it)
+  const int lo = max(cell_start[c] + sc.shift, sc.slot_off);
+  const int hi = min(cell_start[c + 1] + sc.shift, sc.slot_off + sc.n);
   int r = 0;
   for (int u = lo; u < hi; ++u) r += (ppid[u] < p) ? 1 : 0;
   const int f = lo + r;
@@ -841,6 +866,17 @@ __global__ __launch_bounds__(256) void k_delta_ref(
   }
 }
 
+// Where the last kernel of a step leaves the next step's predicted
+// positions, cell keys and arrival ranks, and adds the cell counts
+// (predict_count_slot on the state it has just written): the next step's
+// sort then starts with the scan.
+struct NextSort {
+  float4 *pred_u;
+  int *key;
+  int *rank;
+  int *count;
+};
+
 // Final write of a slot: x = x*, v, |v| (update_positions.cl:9-12), id.
 __device__ __forceinline__ void write_final(int i, float3 xs, float4 v,
                                             float len, float4 *__restrict__ pos,
@@ -863,7 +899,7 @@ __global__ __launch_bounds__(256) void k_xsph_final_ref(
     const float4 *__restrict__ X, const int *__restrict__ cs,
     const float4 *__restrict__ V, const int *__restrict__ pid_s,
     float4 *__restrict__ pos, float4 *__restrict__ vel,
-    int *__restrict__ pid, int *__restrict__ err) {
+    int *__restrict__ pid, int *__restrict__ err, NextSort ns) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
@@ -879,6 +915,8 @@ __global__ __launch_bounds__(256) void k_xsph_final_ref(
   float len;
   const float4 v = add_len_exact(vi, dv.x, dv.y, dv.z, len);
   write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+  predict_count_slot<kRef>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len), v,
+                           1, ns.pred_u, ns.key, ns.rank, ns.count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1129,7 +1167,8 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride,
     const int *__restrict__ pid_s, float4 *__restrict__ pos,
-    float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err) {
+    float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err,
+    NextSort ns) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -1158,9 +1197,12 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
     sz += (vj.z - vi.z) * w;
   });
   const float c = sc.c_xsph * sc.poly6;
-  const float4 v = make_float4(vi.x + c * sx, vi.y + c * sy, vi.z + c * sz, vi.w);
+  const float4 v =
+      make_float4(vi.x + c * sx, vi.y + c * sy, vi.z + c * sz, vi.w);
   const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
   write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+  predict_count_slot<kPaper>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len),
+                             v, 1, ns.pred_u, ns.key, ns.rank, ns.count);
 }
 
 // XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15) over
@@ -1228,7 +1270,7 @@ __global__ __launch_bounds__(256) void k_vort_final(
     const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, const int *__restrict__ pid_s,
     float4 *__restrict__ pos, float4 *__restrict__ vel, int *__restrict__ pid,
-    int *__restrict__ err) {
+    int *__restrict__ err, NextSort ns) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -1265,6 +1307,8 @@ __global__ __launch_bounds__(256) void k_vort_final(
   }
   const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
   write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+  predict_count_slot<kPaper>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len),
+                             v, 1, ns.pred_u, ns.key, ns.rank, ns.count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1434,14 +1478,19 @@ struct Launcher {
                   (size_t)(lds_ints) * (L).sh.block[cls] * sizeof(int),  \
                   __VA_ARGS__)
 
+// The sort of a step.  with_predict: predict, bin and count here (probes,
+// diagnostics); otherwise the previous step's last kernel (or
+// enqueue_predict) has done it and the counts are in place.
 template <int PH>
-static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
+static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
+                       bool with_predict) {
   const StepShape &sh = L.sh;
   const int bs = 256;
   const int grid = sh.slots / bs;
-  PBF_LAUNCH(L, kClsPredict, k_predict_count<PH>, grid, bs, d.scenes,
-             d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank, d.count,
-             predict);
+  if (with_predict)
+    PBF_LAUNCH(L, kClsPredict, k_predict_count<PH>, grid, bs, d.scenes,
+               d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank, d.count,
+               predict);
   PBF_LAUNCH(L, kClsScan, k_scan_reduce, sh.scan_blocks, 256,
              reinterpret_cast<const int4 *>(d.count), d.partial);
   PBF_LAUNCH(L, kClsScan, k_scan_top, 1, 256, d.partial, sh.scan_blocks);
@@ -1449,15 +1498,19 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict) {
              reinterpret_cast<int4 *>(d.count), d.partial,
              reinterpret_cast<int4 *>(d.cell_start));
   PBF_LAUNCH(L, kClsScatter, k_scatter, grid, bs, d.scenes, d.block_scene,
-             d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid);
+             d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid, d.err);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
              d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
              d.xa, d.xold, sh.fp16_iterations > 0 ? d.kc : nullptr);
 }
 
+static NextSort next_sort(const DevBuffers &d) {
+  return NextSort{d.pred_u, d.key, d.rank, d.count};
+}
+
 static void step_ref(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
-  sort_stage<kRef>(d, L, 1);
+  sort_stage<kRef>(d, L, 1, false);
   float4 *cur = d.xa, *nxt = d.xb;
   for (int it = 0; it < sh.iterations; ++it) {
     PBF_GATHER(L, kClsLambda, k_lambda_ref, kRefQueue, d.scenes,
@@ -1472,7 +1525,8 @@ static void step_ref(const DevBuffers &d, Launcher &L) {
     std::swap(cur, nxt);
   }
   PBF_GATHER(L, kClsXsph, k_xsph_final_ref, 0, d.scenes, d.block_scene, cur,
-             d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err);
+             d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err,
+             next_sort(d));
 }
 
 // A delta pass, at the s_corr exponent instance of the step's scenes.
@@ -1491,7 +1545,7 @@ static void delta_pass(const DevBuffers &d, Launcher &L) {
 
 static void step_paper(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
-  sort_stage<kPaper>(d, L, 1);
+  sort_stage<kPaper>(d, L, 1, false);
   const float4 *XS = d.xa;
   const int P = sh.slots;
   // iteration 0's lambda pass is the FP32 search; mixed precision runs the
@@ -1519,12 +1573,12 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
     PBF_GATHER(L, kClsXsph, k_xsph_omega, 9, d.scenes, d.block_scene, XS,
                d.xv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx, d.om, d.xb);
     PBF_GATHER(L, kClsVort, k_vort_final, 9, d.scenes, d.block_scene, XS,
-               d.xb, d.cell_start, d.vx, d.om, d.nl, d.ncnt, d.rbase, P, d.pid_s,
-               d.pos, d.vel, d.pid, d.err);
+               d.xb, d.cell_start, d.vx, d.om, d.nl, d.ncnt, d.rbase, P,
+               d.pid_s, d.pos, d.vel, d.pid, d.err, next_sort(d));
   } else {
     PBF_GATHER(L, kClsXsph, k_xsph_final_paper, 9, d.scenes, d.block_scene,
                XS, d.xv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.pid_s, d.pos,
-               d.vel, d.pid, d.err);
+               d.vel, d.pid, d.err, next_sort(d));
   }
 }
 
@@ -1577,17 +1631,39 @@ bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
   return true;
 }
 
+// Zero the cell counts (outside graphs only: a memset node is not captured).
+static void clear_counts(const DevBuffers &d, const StepShape &sh,
+                         hipStream_t st) {
+  (void)hipMemsetAsync(d.count, 0,
+                       (size_t)sh.scan_blocks * kScanTile * sizeof(int), st);
+}
+
+void enqueue_predict(const DevBuffers &d, const StepShape &sh,
+                     hipStream_t st) {
+  clear_counts(d, sh, st);
+  const int bs = 256, grid = sh.slots / bs;
+  if (sh.physics == kRef)
+    k_predict_count<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.pos,
+                                               d.vel, d.pred_u, d.key, d.rank,
+                                               d.count, 1);
+  else
+    k_predict_count<kPaper><<<grid, bs, 0, st>>>(
+        d.scenes, d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank,
+        d.count, 1);
+}
+
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
   Launcher L{sh, st, nullptr};
   const int bs = 256, grid = sh.slots / bs;
+  clear_counts(d, sh, st);
   if (sh.physics == kRef) {
-    sort_stage<kRef>(d, L, 1);
+    sort_stage<kRef>(d, L, 1, true);
     k_count_nbrs<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                             d.cell_start, d.nbr);
     k_lambda_ref<<<grid, bs, kRefQueue * bs * sizeof(int), st>>>(
         d.scenes, d.block_scene, d.xa, d.cell_start, d.lam, d.rho);
   } else {
-    sort_stage<kPaper>(d, L, 1);
+    sort_stage<kPaper>(d, L, 1, true);
     k_count_nbrs<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                               d.cell_start, d.nbr);
     k_lambda_paper<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
@@ -1602,12 +1678,13 @@ void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   double *out_dev) {
   Launcher L{sh, st, nullptr};
   const int bs = 256, grid = sh.slots / bs;
+  clear_counts(d, sh, st);
   if (sh.physics == kRef) {
-    sort_stage<kRef>(d, L, 0);
+    sort_stage<kRef>(d, L, 0, true);
     k_diag<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                       d.cell_start, d.vel, d.red);
   } else {
-    sort_stage<kPaper>(d, L, 0);
+    sort_stage<kPaper>(d, L, 0, true);
     k_diag<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                         d.cell_start, d.vel, d.red);
   }

@@ -51,11 +51,22 @@ struct KernelTimer {
   virtual ~KernelTimer() = default;
 };
 
+// One step.  Its sort starts with the scan: the predicted positions, cell
+// keys, arrival ranks and cell counts of the current state must be in place,
+// left by the previous step's last kernel or by enqueue_predict, and the step
+// leaves those of its result in place in turn.
 void enqueue_step(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   KernelTimer *timer);
+// Zero the cell counts and predict, bin and count the current state (for
+// the first step, and after anything that ran its own sort or replaced the
+// state).  Not for graph capture.
+void enqueue_predict(const DevBuffers &d, const StepShape &sh,
+                     hipStream_t st);
 bool repeatable_class(const StepShape &sh, int cls);
 bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                     int cls, int reps);
+// Probe and diagnostics sort the state themselves (zeroing the counts
+// first) and leave the counts zeroed: enqueue_predict before the next step.
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st);
 void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   double *out_dev);
