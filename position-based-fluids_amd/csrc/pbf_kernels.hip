@@ -528,20 +528,39 @@ __global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ partial,
   }
 }
 
-// cell_start = exclusive scan of the counts; the counts are zeroed for the
-// next step.
+// cell_start = exclusive scan of the counts, tile by tile from the tile's
+// offset; the counts are zeroed for the next step.
+__device__ __forceinline__ void scan_down_tile(int4 *__restrict__ count,
+                                               int4 *__restrict__ cell_start,
+                                               int offset, int *lds) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int4 c = count[t];
+  int tot;
+  const int run = block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot) + offset;
+  cell_start[t] = make_int4(run, run + c.x, run + c.x + c.y,
+                            run + c.x + c.y + c.z);
+  count[t] = make_int4(0, 0, 0, 0);
+}
+
+// After k_scan_top: partial holds the exclusive scan of the tile totals.
 __global__ __launch_bounds__(256) void k_scan_down(
     int4 *__restrict__ count, const int *__restrict__ partial,
     int4 *__restrict__ cell_start) {
   __shared__ int lds[4];
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  const int4 c = count[t];
-  int tot;
-  const int run = block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot) +
-                  partial[blockIdx.x];
-  cell_start[t] = make_int4(run, run + c.x, run + c.x + c.y,
-                            run + c.x + c.y + c.z);
-  count[t] = make_int4(0, 0, 0, 0);
+  scan_down_tile(count, cell_start, partial[blockIdx.x], lds);
+}
+
+// Straight after k_scan_reduce (up to kScanSumTiles tiles): partial holds
+// the tile totals, and each tile sums those before it.
+__global__ __launch_bounds__(256) void k_scan_down_sum(
+    int4 *__restrict__ count, const int *__restrict__ partial,
+    int4 *__restrict__ cell_start) {
+  __shared__ int lds[4];
+  int before = 0;
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) before += partial[k];
+  int offset;
+  block_excl_scan256(before, lds, offset);
+  scan_down_tile(count, cell_start, offset, lds);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(
@@ -1493,10 +1512,16 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
                predict);
   PBF_LAUNCH(L, kClsScan, k_scan_reduce, sh.scan_blocks, 256,
              reinterpret_cast<const int4 *>(d.count), d.partial);
-  PBF_LAUNCH(L, kClsScan, k_scan_top, 1, 256, d.partial, sh.scan_blocks);
-  PBF_LAUNCH(L, kClsScan, k_scan_down, sh.scan_blocks, 256,
-             reinterpret_cast<int4 *>(d.count), d.partial,
-             reinterpret_cast<int4 *>(d.cell_start));
+  if (sh.scan_blocks <= kScanSumTiles) {
+    PBF_LAUNCH(L, kClsScan, k_scan_down_sum, sh.scan_blocks, 256,
+               reinterpret_cast<int4 *>(d.count), d.partial,
+               reinterpret_cast<int4 *>(d.cell_start));
+  } else {
+    PBF_LAUNCH(L, kClsScan, k_scan_top, 1, 256, d.partial, sh.scan_blocks);
+    PBF_LAUNCH(L, kClsScan, k_scan_down, sh.scan_blocks, 256,
+               reinterpret_cast<int4 *>(d.count), d.partial,
+               reinterpret_cast<int4 *>(d.cell_start));
+  }
   PBF_LAUNCH(L, kClsScatter, k_scatter, grid, bs, d.scenes, d.block_scene,
              d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid, d.err);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,

@@ -9,13 +9,17 @@ namespace pbf {
 constexpr int kRef = 0;
 constexpr int kPaper = 1;
 
-// scan tiling: 256 threads x 16 counts per block
 // Cell-count scan: 256 work-items x 4 counts (one int4) per work-group.
 // The count and cell_start arrays are allocated to a whole number of tiles
 // of at least M + 1 entries (the padding counts stay 0), so the scan needs
 // no bounds tests and writes cell_start[M] = total by itself.
 constexpr int kScanIpt = 4;
 constexpr int kScanTile = 256 * kScanIpt;
+// Up to this many tiles, each tile of the down-sweep sums the tile totals
+// before it itself (two launches); beyond, a one-work-group scan of the
+// tile totals runs in between (three launches), since the summing work
+// grows with the square of the tile count.
+constexpr int kScanSumTiles = 4096;
 
 // kernel classes reported by the profiler
 enum KernelClass {

@@ -759,11 +759,33 @@ def test_full_size_c2_properties():
     assert np.array_equal(np.sort(ids), np.arange(c2.n))
     assert np.all(np.diff(g["cell_of"][ids]) >= 0)
     cs = g["cell_start"]
-    assert cs[0] == 0 and cs[-1] == c2.n and np.all(np.diff(cs) >= 0)
+    counts = np.bincount(g["cell_of"], minlength=cs.size - 1)
+    assert cs[0] == 0 and np.array_equal(cs[1:], np.cumsum(counts))
     with pbf.Simulation(p, parts) as s2:
         s2.step(3)
         pos2, vel2 = s2.dump()
     assert np.array_equal(pos, pos2) and np.array_equal(vel, vel2)
+
+
+@pytest.mark.parametrize("box_x, tiles", [(16.0, 4001), (16.8, 4201)])
+def test_cell_table_on_both_scan_paths(box_x, tiles):
+    """The cell table is the exclusive scan of the per-cell counts on both
+    sides of the two-launch scan's limit of 4096 tiles of 1024 cells: a
+    16 x 8 x 8 box has 640 x 80 x 80 fine cells (4001 tiles), a 16.8 x 8 x 8
+    box 672 x 80 x 80 (4201 tiles, three launches)."""
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(box_x, 8.0, 8.0),
+                             physics=pbf.PHYSICS_PAPER, seed=3)
+    say("scan paths", box_x, tiles)
+    with pbf.Simulation(sc.params[0], sc.particles[0]) as s:
+        s.step(2)
+        m = s.num_cells()
+        g = s.probe()
+    assert (m + 1 + 1023) // 1024 == tiles
+    cs = g["cell_start"]
+    counts = np.bincount(g["cell_of"], minlength=m)
+    assert cs[0] == 0 and np.array_equal(cs[1:], np.cumsum(counts))
+    ids = g["sorted_ids"]
+    assert np.all(np.diff(g["cell_of"][ids]) >= 0)
 
 
 def test_full_size_c3_batch_sampled_scenes():
