@@ -367,7 +367,9 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   const size_t Mpad = (size_t)sh.scan_blocks * pbf::kScanTile;  // >= M + 1
   d.count = S.alloc<int>(Mpad);
   d.cell_start = S.alloc<int>(Mpad);
-  d.partial = S.alloc<int>((size_t)sh.scan_blocks + 1);
+  const size_t Tpad = ((size_t)sh.scan_blocks + pbf::kTopChunk) /
+                      pbf::kTopChunk * pbf::kTopChunk;  // whole scan rounds
+  d.partial = S.alloc<int>(Tpad);
   d.perm = S.alloc<int>(P);
   d.ppid = S.alloc<int>(P);
   d.pid_s = S.alloc<int>(P);
@@ -423,6 +425,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   PBF_HIP(hipMemcpy(d.pid, hpid.data(), P * sizeof(int),
                     hipMemcpyHostToDevice));
   PBF_HIP(hipMemset(d.count, 0, Mpad * sizeof(int)));
+  PBF_HIP(hipMemset(d.partial, 0, Tpad * sizeof(int)));
   PBF_HIP(hipMemset(d.err, 0, sizeof(int)));
   PBF_HIP(hipMemcpy(d.block_scene, S.block_scene.data(),
                     S.block_scene.size() * sizeof(int),

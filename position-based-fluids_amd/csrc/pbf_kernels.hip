@@ -470,8 +470,10 @@ __global__ __launch_bounds__(256) void k_predict_count(
                          count);
 }
 
-// Exclusive scan over a 256-thread block; *total receives the block sum.
-__device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
+// Exclusive scan over a block of W waves (lds: W ints); *total receives
+// the block sum.
+template <int W>
+__device__ __forceinline__ int block_excl_scan(int v, int *lds, int &total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int x = v;
 #pragma unroll
@@ -483,7 +485,7 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
   __syncthreads();
   int before = 0, tot = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < W; ++k) {
     const int t = lds[k];
     before += (k < w) ? t : 0;
     tot += t;
@@ -491,6 +493,10 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
   __syncthreads();
   total = tot;
   return before + x - v;
+}
+
+__device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
+  return block_excl_scan<4>(v, lds, total);
 }
 
 __global__ __launch_bounds__(256) void k_scan_reduce(
@@ -502,26 +508,35 @@ __global__ __launch_bounds__(256) void k_scan_reduce(
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// One block: exclusive scan of the nb block sums in place.
-__global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ partial,
-                                                  int nb) {
-  __shared__ int lds[4];
+// One work-group of kTopThreads: exclusive scan of the nb tile totals in
+// place, kTopIpt consecutive totals per work-item per round.  partial is
+// allocated to whole rounds (its padding stays 0: only entries below nb are
+// written), so the loads need no bounds tests.
+__global__ __launch_bounds__(kTopThreads) void k_scan_top(
+    int *__restrict__ partial, int nb) {
+  __shared__ int lds[kTopThreads / 64];
+  static_assert(kTopIpt == 16, "four int4 per work-item");
   int carry = 0;
-  for (int base = 0; base < nb; base += 256 * kScanIpt) {
-    int v[kScanIpt];
+  for (int base = 0; base < nb; base += kTopChunk) {
+    const int first = base + threadIdx.x * kTopIpt;
+    const int4 *p4 = reinterpret_cast<const int4 *>(partial + first);
+    int v[kTopIpt];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int4 a = p4[k];
+      v[4 * k] = a.x;
+      v[4 * k + 1] = a.y;
+      v[4 * k + 2] = a.z;
+      v[4 * k + 3] = a.w;
+    }
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanIpt; ++k) {
-      const int idx = base + threadIdx.x * kScanIpt + k;
-      v[k] = (idx < nb) ? partial[idx] : 0;
-      s += v[k];
-    }
+    for (int k = 0; k < kTopIpt; ++k) s += v[k];
     int tot;
-    int run = block_excl_scan256(s, lds, tot) + carry;
+    int run = block_excl_scan<kTopThreads / 64>(s, lds, tot) + carry;
 #pragma unroll
-    for (int k = 0; k < kScanIpt; ++k) {
-      const int idx = base + threadIdx.x * kScanIpt + k;
-      if (idx < nb) partial[idx] = run;
+    for (int k = 0; k < kTopIpt; ++k) {
+      if (first + k < nb) partial[first + k] = run;
       run += v[k];
     }
     carry += tot;
@@ -529,22 +544,21 @@ __global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ partial,
 }
 
 // cell_start = exclusive scan of the counts, tile by tile from the tile's
-// offset; the counts are zeroed for the next step.
-__device__ __forceinline__ void scan_down_tile(int4 *__restrict__ count,
-                                               int4 *__restrict__ cell_start,
-                                               int offset, int *lds) {
+// offset (k_scatter zeroes the counts of the occupied cells afterwards).
+__device__ __forceinline__ void scan_down_tile(
+    const int4 *__restrict__ count, int4 *__restrict__ cell_start, int offset,
+    int *lds) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int4 c = count[t];
   int tot;
   const int run = block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot) + offset;
   cell_start[t] = make_int4(run, run + c.x, run + c.x + c.y,
                             run + c.x + c.y + c.z);
-  count[t] = make_int4(0, 0, 0, 0);
 }
 
 // After k_scan_top: partial holds the exclusive scan of the tile totals.
 __global__ __launch_bounds__(256) void k_scan_down(
-    int4 *__restrict__ count, const int *__restrict__ partial,
+    const int4 *__restrict__ count, const int *__restrict__ partial,
     int4 *__restrict__ cell_start) {
   __shared__ int lds[4];
   scan_down_tile(count, cell_start, partial[blockIdx.x], lds);
@@ -553,7 +567,7 @@ __global__ __launch_bounds__(256) void k_scan_down(
 // Straight after k_scan_reduce (up to kScanSumTiles tiles): partial holds
 // the tile totals, and each tile sums those before it.
 __global__ __launch_bounds__(256) void k_scan_down_sum(
-    int4 *__restrict__ count, const int *__restrict__ partial,
+    const int4 *__restrict__ count, const int *__restrict__ partial,
     int4 *__restrict__ cell_start) {
   __shared__ int lds[4];
   int before = 0;
@@ -563,16 +577,21 @@ __global__ __launch_bounds__(256) void k_scan_down_sum(
   scan_down_tile(count, cell_start, offset, lds);
 }
 
+// Also zeroes the count of every occupied cell for the next step's count
+// (the scan has read them; cells without a particle hold 0 already).
 __global__ __launch_bounds__(256) void k_scatter(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const int *__restrict__ key, const int *__restrict__ rank,
     const int *__restrict__ pid, const int *__restrict__ cell_start,
-    int *__restrict__ perm, int *__restrict__ ppid, int *__restrict__ err) {
+    int *__restrict__ count, int *__restrict__ perm, int *__restrict__ ppid,
+    int *__restrict__ err) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const int slot = cell_start[key[i]] + rank[i] + sc.shift;
+  const int k = key[i];
+  count[k] = 0;
+  const int slot = cell_start[k] + rank[i] + sc.shift;
   // counts that disagree with the keys would place a slot outside its
   // scene: reported (error bit 2), never written
   if ((unsigned)(slot - sc.slot_off) >= (unsigned)sc.n) {
@@ -1514,16 +1533,18 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
              reinterpret_cast<const int4 *>(d.count), d.partial);
   if (sh.scan_blocks <= kScanSumTiles) {
     PBF_LAUNCH(L, kClsScan, k_scan_down_sum, sh.scan_blocks, 256,
-               reinterpret_cast<int4 *>(d.count), d.partial,
+               reinterpret_cast<const int4 *>(d.count), d.partial,
                reinterpret_cast<int4 *>(d.cell_start));
   } else {
-    PBF_LAUNCH(L, kClsScan, k_scan_top, 1, 256, d.partial, sh.scan_blocks);
+    PBF_LAUNCH(L, kClsScan, k_scan_top, 1, kTopThreads, d.partial,
+               sh.scan_blocks);
     PBF_LAUNCH(L, kClsScan, k_scan_down, sh.scan_blocks, 256,
-               reinterpret_cast<int4 *>(d.count), d.partial,
+               reinterpret_cast<const int4 *>(d.count), d.partial,
                reinterpret_cast<int4 *>(d.cell_start));
   }
   PBF_LAUNCH(L, kClsScatter, k_scatter, grid, bs, d.scenes, d.block_scene,
-             d.key, d.rank, d.pid, d.cell_start, d.perm, d.ppid, d.err);
+             d.key, d.rank, d.pid, d.cell_start, d.count, d.perm, d.ppid,
+             d.err);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
              d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
              d.xa, d.xold, sh.fp16_iterations > 0 ? d.kc : nullptr);

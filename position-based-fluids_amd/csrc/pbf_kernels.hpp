@@ -20,6 +20,11 @@ constexpr int kScanTile = 256 * kScanIpt;
 // tile totals runs in between (three launches), since the summing work
 // grows with the square of the tile count.
 constexpr int kScanSumTiles = 4096;
+// The one-work-group scan of the tile totals: kTopThreads work-items x
+// kTopIpt totals per round; the totals array is allocated to whole rounds.
+constexpr int kTopThreads = 1024;
+constexpr int kTopIpt = 16;
+constexpr int kTopChunk = kTopThreads * kTopIpt;
 
 // kernel classes reported by the profiler
 enum KernelClass {
