@@ -516,28 +516,43 @@ __global__ __launch_bounds__(kTopThreads) void k_scan_top(
     int *__restrict__ partial, int nb) {
   __shared__ int lds[kTopThreads / 64];
   static_assert(kTopIpt == 16, "four int4 per work-item");
-  int carry = 0;
-  for (int base = 0; base < nb; base += kTopChunk) {
-    const int first = base + threadIdx.x * kTopIpt;
-    const int4 *p4 = reinterpret_cast<const int4 *>(partial + first);
-    int v[kTopIpt];
+  int4 *p4 = reinterpret_cast<int4 *>(partial) + threadIdx.x * 4;
+  constexpr int kChunk4 = kTopChunk / 4;  // int4 per round
+  // the next round's totals are loaded before this round is scanned
+  int4 nxt[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int4 a = p4[k];
-      v[4 * k] = a.x;
-      v[4 * k + 1] = a.y;
-      v[4 * k + 2] = a.z;
-      v[4 * k + 3] = a.w;
+  for (int k = 0; k < 4; ++k) nxt[k] = p4[k];
+  int carry = 0;
+  for (int base = 0; base < nb; base += kTopChunk, p4 += kChunk4) {
+    int4 cur[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
+    if (base + kTopChunk < nb) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) nxt[k] = p4[kChunk4 + k];
     }
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kTopIpt; ++k) s += v[k];
+    for (int k = 0; k < 4; ++k) s += cur[k].x + cur[k].y + cur[k].z + cur[k].w;
     int tot;
     int run = block_excl_scan<kTopThreads / 64>(s, lds, tot) + carry;
+    const int first = base + threadIdx.x * kTopIpt;
 #pragma unroll
-    for (int k = 0; k < kTopIpt; ++k) {
-      if (first + k < nb) partial[first + k] = run;
-      run += v[k];
+    for (int k = 0; k < 4; ++k) {
+      int4 o;
+      o.x = run;
+      o.y = o.x + cur[k].x;
+      o.z = o.y + cur[k].y;
+      o.w = o.z + cur[k].z;
+      run = o.w + cur[k].w;
+      const int idx = first + 4 * k;
+      if (idx + 3 < nb) {
+        p4[k] = o;
+      } else {
+        const int vals[4] = {o.x, o.y, o.z, o.w};
+        for (int q = 0; q < 4; ++q)
+          if (idx + q < nb) partial[idx + q] = vals[q];
+      }
     }
     carry += tot;
   }
