@@ -4,7 +4,8 @@ and the mathematics fix, so that the oracle can then check the HIP path.
 The reference ships no golden vectors or tests (its only fixtures are the
 input scenarios, copied under tests/golden/).  What pins the oracle here:
   * the kernel constants against the -D strings src/main.cpp:149-168
-    produces for h = 0.1 (reproduced in the survey of the reference);
+    produces for h = 0.1 (6-significant-digit formatting of the closed
+    forms);
   * perfect-lattice closed forms of rho and lambda in both physics modes;
   * the two-particle scene of assets/scenarios/two.{par,in}, reduced to a
     scalar recurrence on the separation and worked in FP64;
@@ -126,7 +127,7 @@ def test_lattice_density_and_lambda_closed_form(physics):
     assert out["lambda"][ci] == pytest.approx(lam, rel=2e-5)
 
 
-def test_survey_lattice_numbers_h01():
+def test_lattice_numbers_h01():
     """h = 0.1: rho (self excluded) = 6511.52, lambda_paper = -2.507166e-5,
     lambda_ref = -1.538864e-6 (closed forms quoted for this lattice)."""
     h = 0.1
