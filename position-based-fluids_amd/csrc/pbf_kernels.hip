@@ -752,7 +752,15 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
         f(j + 2, a2, re);
         f(j + 3, a3, re);
       }
-      for (; j < e; ++j) f(j, X[j], re);
+      if (j < e) {
+        // the 1 to 3 slots left, loaded together (a slot past e is loaded
+        // but not visited; past the array a load returns 0) rather than
+        // one load in flight at a time
+        const float4 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2];
+        f(j, a0, re);
+        if (j + 1 < e) f(j + 1, a1, re);
+        if (j + 2 < e) f(j + 2, a2, re);
+      }
     }
   }
 }
