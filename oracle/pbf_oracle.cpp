@@ -28,7 +28,12 @@
 // both modes, the two-particle scene of assets/scenarios/two.in worked by
 // hand, brute-force O(N^2) neighbour counts, antisymmetry of the pressure
 // term (sum of dp = 0 away from walls), stable-sort permutation, clamp
-// cases, FP32-vs-FP64 agreement.
+// cases, FP32-vs-FP64 agreement.  PAPER-only terms, which the reference does
+// not have: s_corr by a two-particle scalar recurrence (n = 2, 4; k = 1e-3,
+// 0.1), the vorticity magnitude of a rigid rotation on the lattice shells,
+// the confinement force on a line of three particles with a known |omega|
+// gradient (orc_vorticity_force_f64), and the sphere projection (inside, at
+// the centre, outside, before the wall clamp).
 
 #include <algorithm>
 #include <cmath>
@@ -721,6 +726,30 @@ int orc_vorticity_f64(const pbf_params *p, int n, const double *pos4,
   build_neighbours(s);
   xsph_vorticity(s);
   std::memcpy(omega4, s.omega.data(), sizeof(double) * 4 * n);
+  return 0;
+}
+
+// PAPER vorticity confinement (eq. 16) alone: the velocity change
+// dt eps (N x omega_i) of vorticity_force for given positions and
+// vorticities omega4 (x, y, z; .w is recomputed as |omega|), FP64, lists
+// built from the positions.
+int orc_vorticity_force_f64(const pbf_params *p, int n, const double *pos4,
+                            const double *omega4, double *dv4) {
+  if (!params_ok(p, n) || !dv4) return 1;
+  pbf_params q = *p;
+  q.physics = PBF_PHYSICS_PAPER;
+  State<double> s;
+  init_state(s, q, n);
+  for (size_t i = 0; i < 4 * (size_t)n; ++i) {
+    s.pos[i] = pos4[i];
+    s.pred[i] = pos4[i];
+    s.omega[i] = omega4[i];
+  }
+  for (int i = 0; i < n; ++i) s.omega[4 * i + 3] = len3(&s.omega[4 * i]);
+  build_lists(s);
+  build_neighbours(s);
+  vorticity_force(s);
+  std::memcpy(dv4, s.dvel.data(), sizeof(double) * 4 * n);
   return 0;
 }
 

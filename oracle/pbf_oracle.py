@@ -41,8 +41,10 @@ def load():
     lib.orc_brute_neighbors_f32.argtypes = [ctypes.c_int, vp, ctypes.c_float, vp]
     lib.orc_density_f64.argtypes = [vp, ctypes.c_int, vp, vp]
     lib.orc_vorticity_f64.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+    lib.orc_vorticity_force_f64.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     for f in ("orc_constants", "orc_run_f32", "orc_run_f64", "orc_probe_f32",
-              "orc_brute_neighbors_f32", "orc_density_f64", "orc_vorticity_f64"):
+              "orc_brute_neighbors_f32", "orc_density_f64", "orc_vorticity_f64",
+              "orc_vorticity_force_f64"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -149,4 +151,15 @@ def vorticity_f64(params, pos4, vel4) -> np.ndarray:
     out = np.zeros((pos.shape[0], 4), np.float64)
     assert load().orc_vorticity_f64(_pp(params), pos.shape[0], _p(pos), _p(vel),
                                     _p(out)) == 0
+    return out
+
+
+def vorticity_force_f64(params, pos4, omega4) -> np.ndarray:
+    """Velocity change of the vorticity confinement force (eq. 16) for the
+    given positions and vorticities (FP64)."""
+    pos = np.ascontiguousarray(pos4, np.float64)
+    om = np.ascontiguousarray(omega4, np.float64)
+    out = np.zeros((pos.shape[0], 4), np.float64)
+    assert load().orc_vorticity_force_f64(_pp(params), pos.shape[0], _p(pos),
+                                          _p(om), _p(out)) == 0
     return out

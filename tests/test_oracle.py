@@ -13,7 +13,12 @@ input scenarios, copied under tests/golden/).  What pins the oracle here:
     sort against the oracle's linked-cell search;
   * antisymmetry of the pressure term (sum of displacements = 0 away from
     walls), mirror symmetry of a lattice, wall / wave-generator clamps,
-    the orientation of the vorticity of a rigid rotation, FP32 vs FP64.
+    the orientation of the vorticity of a rigid rotation, FP32 vs FP64;
+  * the PAPER-only terms against closed forms of Macklin & Mueller's
+    equations: s_corr through a two-particle scalar recurrence, the
+    vorticity magnitude of a rigid rotation over the lattice shells, the
+    confinement force on three particles with a known |omega| gradient, and
+    the obstacle projection with its order relative to the wall clamp.
 """
 import math
 import os
@@ -128,22 +133,31 @@ def test_lattice_density_and_lambda_closed_form(physics):
 
 
 def test_lattice_numbers_h01():
-    """h = 0.1: rho (self excluded) = 6511.52, lambda_paper = -2.507166e-5,
-    lambda_ref = -1.538864e-6 (closed forms quoted for this lattice)."""
+    """h = 0.1: rho (self excluded) = 6511.52 and lambda_paper = -2.507166e-5
+    at the centre of the lattice.  The spacing 0.05 is not a binary fraction,
+    so three of the six sites at q = h (those on the + side of each axis)
+    land just inside the support in FP32 (q = 0.0999999): the centre has 29
+    neighbours, counted here by brute force on the FP32 coordinates, and the
+    three extra pairs add (h^2 - q^2)^3 ~ 1e-21 to rho and (h - q)^4 ~ 1e-28
+    to the lambda denominator, so the closed forms of the 26-neighbour
+    interior still hold to FP32 rounding."""
     h = 0.1
     poly6 = 315.0 / (64.0 * math.pi * h ** 9)
     rho = sum(c * poly6 * (h * h - f * h * h) ** 3 for c, f in SHELLS)
     assert rho == pytest.approx(6511.5199, rel=1e-6)
-    # evaluate the oracle on such a lattice (binary-exact coordinates are not
-    # available at h = 0.1; the jitter-free lattice keeps q = h pairs just
-    # outside or inside the support, so use the 26-neighbour interior)
     p = params(box=(4.0, 4.0, 4.0), n=(40, 40, 40), physics=pbf.PHYSICS_PAPER,
                gravity_y=0.0)
     pos, vel, ci = _lattice_state(h=0.1, n=9, origin=1.0)
     out = orc.probe_f32(p, pos, vel)
-    if out["nbr_count"][ci] == 26:
-        assert out["rho"][ci] == pytest.approx(6511.5199, rel=1e-5)
-        assert out["lambda"][ci] == pytest.approx(-2.507166e-5, rel=1e-4)
+    # brute force on the FP32 coordinates: r^2 left to right, 0 < r^2 < h^2
+    r = (pos[ci, :3] - pos[:, :3]).astype(np.float32)
+    r2 = (r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2]
+    inside = (r2 > 0) & (r2 < np.float32(orc.constants(p)["h2"]))
+    q = np.sqrt(r2[inside].astype(np.float64))
+    assert inside.sum() == 29 and (np.abs(q - h) < 1e-6).sum() == 3
+    assert out["nbr_count"][ci] == 29
+    assert out["rho"][ci] == pytest.approx(6511.5199, rel=1e-5)
+    assert out["lambda"][ci] == pytest.approx(-2.507166e-5, rel=1e-4)
 
 
 def test_lattice_centre_stays_put_fp64():
@@ -359,3 +373,215 @@ def test_ref_xsph_divides_by_a_zero_density():
     q = params(physics=pbf.PHYSICS_PAPER, gravity_y=0.0, iterations=1)
     x, v = orc.run_f64(q, pos.astype(np.float64), vel.astype(np.float64), 1)
     assert np.isfinite(x).all() and np.isfinite(v).all()
+
+
+# ----------------------------------------------------------------------------
+# PAPER-only physics: s_corr, vorticity, vorticity confinement, obstacle.
+# The reference has none of these (s_corr is a commented sketch at
+# src/kernels/compute_delta.cl:65-76, apply_vorticity_and_viscosity.cl only
+# does XSPH, the clamp of compute_delta.cl:128-150 knows only walls), so they
+# are pinned against closed forms of Macklin & Mueller's equations.
+# ----------------------------------------------------------------------------
+
+def _paper_pair(d, k, n, iterations):
+    """Two particles a distance d apart along x at the centre of a 2 m box,
+    at rest, no gravity: PAPER physics with s_corr (k, n, dq = 0.2h)."""
+    p = params(box=(2.0, 2.0, 2.0), n=(20, 20, 20), physics=pbf.PHYSICS_PAPER,
+               gravity_y=0.0, iterations=iterations, scorr_k=k,
+               scorr_n=float(n))
+    pos = np.zeros((2, 4))
+    vel = np.zeros((2, 4))
+    pos[0, :3] = [1.0 + d / 2, 1.03, 0.97]
+    pos[1, :3] = [1.0 - d / 2, 1.03, 0.97]
+    vel[:, 3] = 1.0
+    return p, pos, vel
+
+
+@pytest.mark.parametrize("iterations", [1, 4])
+@pytest.mark.parametrize("n", [2, 4])
+@pytest.mark.parametrize("k", [1e-3, 0.1])
+def test_scorr_two_particle_recurrence(k, n, iterations):
+    """s_corr = -k (W(d) / W(dq))^n added to lambda_i + lambda_j (eqs. 13,
+    14), reduced to a scalar recurrence on the separation d: with
+    g = spiky (h - d)^2 the gradient magnitude, each iteration moves each
+    particle by (2 lambda + s_corr) g / rho0 toward the other, lambda from the
+    paper's denominator (|sum grad W|^2 + sum |grad W|^2)/rho0^2 + eps =
+    2 g^2 / rho0^2 + eps.  Then v = (x* - x)/dt, XSPH with the density of the
+    last lambda pass, and no vorticity (v parallel to r)."""
+    h = 0.1
+    poly6 = 315.0 / (64.0 * math.pi * h ** 9)
+    spiky = 45.0 / (math.pi * h ** 6)
+    rho0, eps, dt = 6378.0, 600.0, float(np.float32(1.0 / 120.0))
+    # the parameters travel as float32 (pbf_params)
+    dq = float(np.float32(0.2)) * h
+    w_dq = poly6 * (h * h - dq * dq) ** 3
+    p, pos, vel = _paper_pair(0.5 * h, k, n, iterations)
+    strong = k == 0.1
+    k = float(np.float32(k))
+    x, v = orc.run_f64(p, pos, vel, 1)
+
+    d0 = pos[0, 0] - pos[1, 0]
+    d = d0
+    rho = 0.0
+    for _ in range(iterations):
+        if d >= h:          # the pair has left the support: no terms
+            rho = 0.0
+            continue
+        w = poly6 * (h * h - d * d) ** 3
+        g = spiky * (h - d) ** 2
+        rho = w
+        lam = -(rho / rho0 - 1.0) / (2.0 * g * g / rho0 ** 2 + eps)
+        scorr = -k * (w / w_dq) ** n
+        d -= 2.0 * (2.0 * lam + scorr) * g / rho0
+    mid = 0.5 * (pos[0, 0] + pos[1, 0])
+    assert x[0, 0] == pytest.approx(mid + d / 2, abs=1e-13)
+    assert x[1, 0] == pytest.approx(mid - d / 2, abs=1e-13)
+    assert np.array_equal(x[:, 1:3], pos[:, 1:3])
+    vx = (x[0, 0] - pos[0, 0]) / dt
+    dvx = 0.0
+    if d < h and rho > 0.0:   # XSPH, weighted by the last pass's density
+        wf = poly6 * (h * h - d * d) ** 3
+        dvx = float(np.float32(0.01)) * (1.0 / rho) * (-2.0 * vx) * wf
+    assert v[0, 0] == pytest.approx(vx + dvx, rel=1e-11)
+    assert v[1, 0] == pytest.approx(-(vx + dvx), rel=1e-11)
+    # what the equations fix whatever the algebra: at d = h/2 the pair is
+    # under-dense (lambda > 0 pulls it together); the artificial pressure
+    # s_corr is repulsive, and at k = 0.1 it outweighs 2 lambda in the first
+    # iteration (later ones start from the wider, more under-dense pair)
+    if iterations == 1:
+        assert (d > d0) if strong else (d < d0)
+
+
+def test_scorr_is_what_separates_the_pair():
+    """The same pair with k = 0 and k = 0.1 (n = 4): the difference of the
+    separations after one iteration is 2 k (W/W(dq))^4 g / rho0 exactly."""
+    h = 0.1
+    poly6 = 315.0 / (64.0 * math.pi * h ** 9)
+    spiky = 45.0 / (math.pi * h ** 6)
+    p0, pos, vel = _paper_pair(0.4 * h, 0.0, 4, 1)
+    p1, _, _ = _paper_pair(0.4 * h, 0.1, 4, 1)
+    x0, _ = orc.run_f64(p0, pos, vel, 1)
+    x1, _ = orc.run_f64(p1, pos, vel, 1)
+    d = 0.4 * h
+    dq = float(np.float32(0.2)) * h
+    q = (h * h - d * d) ** 3 / (h * h - dq * dq) ** 3
+    expect = 2.0 * float(np.float32(0.1)) * q ** 4 * spiky * (h - d) ** 2 / 6378.0
+    sep0 = x0[0, 0] - x0[1, 0]
+    sep1 = x1[0, 0] - x1[1, 0]
+    assert sep1 - sep0 == pytest.approx(expect, rel=1e-9)
+    assert poly6 > 0  # W(d)/W(dq) needs no poly6: the constant cancels
+
+
+def test_vorticity_magnitude_of_rigid_rotation():
+    """omega_i = sum_j (v_j - v_i) x grad_j W_ij (eq. 15) for the rigid
+    rotation v = Omega x (x - c) at the centre of a perfect lattice (h = 1/8,
+    binary-exact sites).  With r = x_i - x_j, v_j - v_i = -Omega x r and
+    grad_j W_ij = spiky (h - q)^2 r/q, so each pair adds
+    spiky (h - q)^2 q (Omega - rhat (rhat . Omega)); over a shell of c_k
+    sites sum rhat rhat^T = (c_k / 3) I, hence
+    omega = (2/3) Omega sum_k c_k spiky (h - q_k)^2 q_k over the shells
+    (6, h/2), (12, h/sqrt 2), (8, sqrt(3) h/2)."""
+    h = 0.125
+    p = params(box=(2.0, 2.0, 2.0), n=(16, 16, 16), physics=pbf.PHYSICS_PAPER)
+    pos, _, ci = _lattice_state(h)
+    c = pos[ci, :3].astype(np.float64)
+    omega = np.array([0.3, -0.2, 1.0])
+    vel = np.zeros((pos.shape[0], 4))
+    vel[:, :3] = np.cross(omega, pos[:, :3] - c)
+    w = orc.vorticity_f64(p, pos.astype(np.float64), vel)[ci]
+    spiky = 45.0 / (math.pi * h ** 6)
+    mag = sum(cnt * spiky * (h - math.sqrt(f) * h) ** 2 * math.sqrt(f) * h
+              for cnt, f in SHELLS)
+    expect = (2.0 / 3.0) * mag * omega
+    assert np.allclose(w[:3], expect, rtol=1e-12, atol=0.0)
+    assert w[3] == pytest.approx(np.linalg.norm(expect), rel=1e-12)
+
+
+def _three_in_a_row(axis, d1, d2, omegas):
+    """Particles A, B, C at B - d1 u, B, B + d2 u along unit vector `axis`
+    (d1 + d2 > h: A and C are not neighbours), vorticities `omegas`."""
+    u = np.asarray(axis, np.float64)
+    u = u / np.linalg.norm(u)
+    b = np.array([1.0, 1.0, 1.0])
+    pos = np.zeros((3, 4))
+    pos[0, :3] = b - d1 * u
+    pos[1, :3] = b
+    pos[2, :3] = b + d2 * u
+    om = np.zeros((3, 4))
+    om[:, :3] = omegas
+    return pos, om, u
+
+
+@pytest.mark.parametrize("axis", [(1.0, 0.0, 0.0), (1.0, 2.0, 2.0)])
+def test_vorticity_force_points_up_the_gradient(axis):
+    """eta = grad |omega| (eq. 16), N = eta/|eta|, dv = dt eps (N x omega_i).
+    On a line of three particles with |omega| growing along u, every
+    particle's gradient estimate points along +u whatever the kernel's
+    magnitudes, so N = u exactly and dv = dt eps (u x omega_i); reversing the
+    growth reverses N, and equal magnitudes give no force."""
+    h = 0.1
+    p = params(box=(2.0, 2.0, 2.0), n=(20, 20, 20), physics=pbf.PHYSICS_PAPER)
+    dt = float(np.float32(1.0 / 120.0))
+    eps = float(np.float32(1e-4))
+    oms = np.array([[0.0, 0.0, 1.0], [0.0, 0.5, 2.0], [0.0, 0.0, 4.0]])
+    pos, om, u = _three_in_a_row(axis, 0.4 * h, 0.7 * h, oms)
+    dv = orc.vorticity_force_f64(p, pos, om)
+    for i in range(3):
+        expect = dt * eps * np.cross(u, oms[i])
+        assert np.allclose(dv[i, :3], expect, rtol=0, atol=1e-12 * np.abs(expect).max())
+    # reversed growth: N = -u
+    rev = oms[::-1].copy()
+    dv = orc.vorticity_force_f64(p, pos, np.pad(rev, ((0, 0), (0, 1))))
+    for i in range(3):
+        expect = dt * eps * np.cross(-u, rev[i])
+        assert np.allclose(dv[i, :3], expect, rtol=0, atol=1e-12 * np.abs(expect).max())
+    # equal magnitudes: eta = 0, no force
+    same = np.array([[0.0, 0.0, 2.0], [0.0, 2.0, 0.0], [2.0, 0.0, 0.0]])
+    dv = orc.vorticity_force_f64(p, pos, np.pad(same, ((0, 0), (0, 1))))
+    assert np.abs(dv[:, :3]).max() < 1e-18
+
+
+def _one_particle_step(at, sphere, box=(2.0, 2.0, 2.0)):
+    """One PAPER step of a single particle at rest without gravity: no
+    neighbours, so dp = 0 and the step reduces to the collision handling."""
+    p = params(box=box, n=(20, 20, 20), physics=pbf.PHYSICS_PAPER, gravity_y=0.0,
+               iterations=1)
+    for a in range(4):
+        p.sphere[a] = sphere[a]
+    pos = np.zeros((1, 4))
+    pos[0, :3] = at
+    vel = np.zeros((1, 4))
+    vel[0, 3] = 1.0
+    x, v = orc.run_f64(p, pos, vel, 1)
+    return x[0, :3], v[0, :3]
+
+
+def test_sphere_obstacle_projection():
+    """A point within r + h of the obstacle's centre moves to distance r + h
+    along its radius; the centre itself moves to +y; a point outside is
+    untouched.  The velocity is the displacement over dt."""
+    dt = float(np.float32(1.0 / 120.0))
+    c = np.array([1.0, 1.0, 1.0])
+    r, h = float(np.float32(0.2)), 0.1   # the radius travels as float32
+    sph = (1.0, 1.0, 1.0, r)
+    inside = np.array([1.1, 1.05, 0.95])
+    x, v = _one_particle_step(inside, sph)
+    expect = c + (inside - c) * (r + h) / np.linalg.norm(inside - c)
+    assert np.allclose(x, expect, rtol=0, atol=1e-15)
+    assert np.linalg.norm(x - c) == pytest.approx(r + h, rel=1e-14)
+    assert np.allclose(v, (x - inside) / dt, rtol=1e-12, atol=0)
+    x, _ = _one_particle_step(c, sph)
+    assert np.array_equal(x, c + np.array([0.0, r + h, 0.0]))
+    outside = np.array([1.5, 1.0, 1.0])
+    x, v = _one_particle_step(outside, sph)
+    assert np.array_equal(x, outside) and np.array_equal(v, np.zeros(3))
+
+
+def test_sphere_projection_runs_before_the_wall_clamp():
+    """Obstacle of radius 0.05 centred 0.2 m above the floor (r + h = 0.15):
+    a particle at y = 0.12 is projected down to y = 0.05, inside the floor
+    band, and the wall clamp then lifts it to min + h = 0.1.  In the other
+    order it would end at 0.05 (the clamp leaves y = 0.12 alone)."""
+    x, _ = _one_particle_step([1.0, 0.12, 1.0], (1.0, 0.2, 1.0, 0.05))
+    assert x[1] == 0.1
+    assert x[0] == 1.0 and x[2] == 1.0
