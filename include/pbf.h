@@ -192,12 +192,15 @@ int pbf_probe(pbf_sim *sim, int32_t *cell_of, int32_t *sorted_ids,
  * that the neighbour search can cut its rows to the reach h). */
 uint32_t pbf_num_cells(const pbf_sim *sim);
 
-/* FP64 reductions on the device over the current state (all scenes):
- * out[0] = max |rho_i/rho0 - 1| over particles with at least one neighbour,
- * out[1] = mean |rho_i/rho0 - 1| over the same particles,
+/* FP64 reductions on the device over the current state (all scenes), with
+ * e_i = rho_i/rho0 - 1, rho_i the density at the current positions (self
+ * excluded; a neighbour is a j != i with 0 < |x_i - x_j|^2 < h^2):
+ * out[0] = max over all particles of max(e_i, 0) (largest compression),
+ * out[1] = mean |e_i| over the particles with at least one neighbour,
  * out[2..4] = total momentum sum m v, out[5] = kinetic energy sum m v^2/2,
- * out[6] = number of particles counted in out[0..1].
- * The density is evaluated at the current positions. */
+ * out[6] = number of particles with at least one neighbour,
+ * out[7] = max |e_i| over the particles with at least one neighbour.
+ * Isolated particles (e_i = -1) enter only out[2..5]. */
 int pbf_diagnostics(pbf_sim *sim, double out[8]);
 
 /* Run nsteps steps, timing every kernel launch with start/stop events

@@ -1,5 +1,6 @@
 // Runner.cpp - headless time loop (src/Runner.cpp:24-160).
 #include "Runner.hpp"
+#include "WaveGenerator.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -53,7 +54,6 @@ std::vector<double> Runner::run(const ConfigParameters &parameters,
   float wave = 0.0f;
   unsigned stepCount = 0;
   std::vector<double> times;
-  const double pi = 3.14159265358979323846;
 
   do {
     const auto t0 = std::chrono::steady_clock::now();
@@ -62,10 +62,7 @@ std::vector<double> Runner::run(const ConfigParameters &parameters,
     if (waves) {
       // Runner.cpp:91-104: x-min wall oscillates over a third of the box
       const float push = (sizesMax.s[0] - sizesMin.s[0]) / 3.0f;
-      const float value =
-          (float)(std::sin(2.0 * pi * 1.0 * wave - pi / 2.0) * push / 2.0f +
-                  push / 2.0f);
-      simulation.setWaveGenerator(value);
+      simulation.setWaveGenerator(waveGeneratorValue(push, wave));
       wave += parameters.timeStepLength;
     }
     const auto t1 = std::chrono::steady_clock::now();

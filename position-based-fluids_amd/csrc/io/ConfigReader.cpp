@@ -8,14 +8,35 @@
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
+#include <vector>
 
 ConfigParameters ConfigReader::read(const std::string &filename) const {
   ConfigParameters p;
   std::ifstream ifs(filename.c_str());
   if (!ifs) throw std::runtime_error("Could not open parameter file!");
 
+  std::vector<std::string> lines;
   std::string line, key;
-  while (std::getline(ifs, line)) {
+  while (std::getline(ifs, line)) lines.push_back(line);
+
+  // The physics mode sets the defaults of the whole physics block, so it is
+  // applied before every other key wherever it stands in the file (the last
+  // one wins); the explicit physics keys then override those defaults.
+  for (const std::string &l : lines) {
+    std::istringstream ss(l);
+    std::string v;
+    if (!(ss >> key) || key != "physics") continue;
+    ss >> v;
+    if (v == "ref")
+      pbf_params_init(&p.physics, PBF_PHYSICS_REF);
+    else if (v == "paper")
+      pbf_params_init(&p.physics, PBF_PHYSICS_PAPER);
+    else
+      throw std::runtime_error("physics must be 'ref' or 'paper'");
+  }
+
+  for (const std::string &l : lines) {
+    line = l;
     if (!line.empty()) {
       std::istringstream ss(line);
       if (!(ss >> key)) {
@@ -57,19 +78,7 @@ ConfigParameters ConfigReader::read(const std::string &filename) const {
       } else if (key == "restdensity") {
         ss >> p.restDensity;
       } else if (key == "physics") {
-        std::string v;
-        ss >> v;
-        if (v == "ref") {
-          const pbf_params old = p.physics;
-          pbf_params_init(&p.physics, PBF_PHYSICS_REF);
-          p.physics.block_size = old.block_size;
-        } else if (v == "paper") {
-          const pbf_params old = p.physics;
-          pbf_params_init(&p.physics, PBF_PHYSICS_PAPER);
-          p.physics.block_size = old.block_size;
-        } else {
-          throw std::runtime_error("physics must be 'ref' or 'paper'");
-        }
+        // applied above, before the other keys
       } else if (key == "iterations") {
         ss >> p.physics.iterations;
       } else if (key == "relaxation") {

@@ -843,7 +843,8 @@ int pbf_diagnostics(pbf_sim *S, double out[8]) {
     PBF_HIP(hipMemcpyAsync(h, S->diag_out, sizeof h, hipMemcpyDeviceToHost,
                            S->stream));
     PBF_HIP(hipStreamSynchronize(S->stream));
-    // out: max (rho/rho0-1)+, mean |rho/rho0-1|, momentum xyz, energy, count
+    // h: max e+, sum |e| (with a neighbour), count (with a neighbour),
+    // momentum xyz, energy, max |e| (with a neighbour)
     out[0] = h[0];
     out[1] = h[2] > 0 ? h[1] / h[2] : 0.0;
     out[2] = h[3];
@@ -851,7 +852,7 @@ int pbf_diagnostics(pbf_sim *S, double out[8]) {
     out[4] = h[5];
     out[5] = h[6];
     out[6] = h[2];
-    out[7] = 0.0;
+    out[7] = h[7];
   });
 }
 

@@ -1410,8 +1410,10 @@ __global__ __launch_bounds__(256) void k_count_nbrs(
   nbr[i] = c;
 }
 
-// Density at the sorted positions X, then per-block FP64 partials:
-// [max (rho/rho0-1)+, sum |rho/rho0-1|, count, px, py, pz, energy, 0].
+// Density at the sorted positions X, then per-block FP64 partials, e =
+// rho/rho0 - 1: [max e+ (all particles), sum |e| and count (particles with
+// at least one neighbour), px, py, pz, energy, max |e| (particles with a
+// neighbour)].
 template <int PH>
 __global__ __launch_bounds__(256) void k_diag(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
@@ -1426,6 +1428,7 @@ __global__ __launch_bounds__(256) void k_diag(
     const float4 x4 = X[i];
     const float3 xi = make_float3(x4.x, x4.y, x4.z);
     float wsum = 0.0f;
+    int pairs = 0;
     for_stencil<PH>(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
       const float4 xj = X[j];
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -1433,27 +1436,29 @@ __global__ __launch_bounds__(256) void k_diag(
       if (j != i && r2 > 0.0f && r2 < sc.h2) {
         const float t = sc.h2 - r2;
         wsum += t * t * t;
+        ++pairs;
       }
     });
     const double e = (double)(sc.poly6 * wsum) / (double)sc.rho0 - 1.0;
     const float4 vv = vel[i];
     const double m = vv.w;
     v[0] = e > 0.0 ? e : 0.0;
-    v[1] = fabs(e);
-    v[2] = 1.0;
+    v[1] = pairs ? fabs(e) : 0.0;
+    v[2] = pairs ? 1.0 : 0.0;
     v[3] = m * vv.x;
     v[4] = m * vv.y;
     v[5] = m * vv.z;
     v[6] = 0.5 * m * ((double)vv.x * vv.x + (double)vv.y * vv.y +
                       (double)vv.z * vv.z);
+    v[7] = v[1];
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < 7; ++k) {
+  for (int k = 0; k < 8; ++k) {
     double x = v[k];
     for (int o = 32; o > 0; o >>= 1) {
       const double y = __shfl_xor(x, o, 64);
-      x = (k == 0) ? fmax(x, y) : x + y;
+      x = (k == 0 || k == 7) ? fmax(x, y) : x + y;
     }
     v[k] = x;
   }
@@ -1464,7 +1469,8 @@ __global__ __launch_bounds__(256) void k_diag(
     const int k = threadIdx.x;
     const int nw = blockDim.x >> 6;
     double x = lds[k][0];
-    for (int q = 1; q < nw; ++q) x = (k == 0) ? fmax(x, lds[k][q]) : x + lds[k][q];
+    for (int q = 1; q < nw; ++q)
+      x = (k == 0 || k == 7) ? fmax(x, lds[k][q]) : x + lds[k][q];
     red[(size_t)lb * 8 + k] = x;
   }
 }
@@ -1478,14 +1484,14 @@ __global__ __launch_bounds__(256) void k_diag_final(double *__restrict__ red,
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
     for (int k = 0; k < 8; ++k) {
       const double x = red[(size_t)b * 8 + k];
-      v[k] = (k == 0) ? fmax(v[k], x) : v[k] + x;
+      v[k] = (k == 0 || k == 7) ? fmax(v[k], x) : v[k] + x;
     }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int k = 0; k < 8; ++k) {
     double x = v[k];
     for (int o = 32; o > 0; o >>= 1) {
       const double y = __shfl_xor(x, o, 64);
-      x = (k == 0) ? fmax(x, y) : x + y;
+      x = (k == 0 || k == 7) ? fmax(x, y) : x + y;
     }
     v[k] = x;
   }
@@ -1495,7 +1501,8 @@ __global__ __launch_bounds__(256) void k_diag_final(double *__restrict__ red,
   if (threadIdx.x < 8) {
     const int k = threadIdx.x;
     double x = lds[k][0];
-    for (int q = 1; q < 4; ++q) x = (k == 0) ? fmax(x, lds[k][q]) : x + lds[k][q];
+    for (int q = 1; q < 4; ++q)
+      x = (k == 0 || k == 7) ? fmax(x, lds[k][q]) : x + lds[k][q];
     out[k] = x;
   }
 }

@@ -280,7 +280,7 @@ class Simulation:
         _check(self._lib.pbf_diagnostics(self._h, d))
         return {"max_compression": d[0], "mean_abs_density_err": d[1],
                 "momentum": (d[2], d[3], d[4]), "kinetic_energy": d[5],
-                "count": d[6]}
+                "count": d[6], "max_abs_density_err": d[7]}
 
     def profile(self, nsteps: int) -> dict:
         cap = 16

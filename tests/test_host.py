@@ -66,6 +66,25 @@ def test_read_par_physics_keys_and_unknown_key(tmp_path):
     assert p.x_max == 2.0 and p.x_n == 20.0
 
 
+def test_read_par_physics_key_order_does_not_matter(tmp_path):
+    """Physics keys given before `physics paper` keep their values; the
+    mode only supplies the defaults of the keys the file does not set."""
+    f = tmp_path / "o.par"
+    f.write_text("x_max 2\nx_n 20\niterations 6\nblock_size 64\n"
+                 "scorr_n 2\nphysics paper\nxsph_c 0.02\n")
+    p, _ = pbf.read_par(str(f))
+    assert p.physics == pbf.PHYSICS_PAPER
+    assert p.iterations == 6 and p.block_size == 64 and p.scorr_n == 2.0
+    assert p.xsph_c == np.float32(0.02)
+    # the unset keys take the PAPER defaults
+    assert p.relaxation == 600.0 and p.scorr_k == np.float32(0.001)
+    assert p.vorticity_eps == np.float32(1e-4)
+    g = tmp_path / "bad.par"
+    g.write_text("iterations 2\nphysics fast\n")
+    with pytest.raises(pbf.PBFError):
+        pbf.read_par(str(g))
+
+
 def test_read_par_missing_file():
     with pytest.raises(pbf.PBFError):
         pbf.read_par("/nonexistent/file.par")
@@ -218,3 +237,47 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=300)
     assert r.returncode != 0
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_wave_generator_sequence_matches_reference_formula(tmp_path):
+    """The Runner's moving-wall offset (csrc/WaveGenerator.hpp), compiled
+    with g++, against a numpy restatement of src/Runner.cpp:91-104: push
+    length (x_max - x_min) / 3.0f and phase -M_PI / 2.0f stored as cl_float,
+    sin(2.0f * M_PI * frequency * wave + phase) evaluated in double, the
+    result stored as cl_float, wave accumulated in float by dt."""
+    src = tmp_path / "wave.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include "WaveGenerator.hpp"\n'
+        "int main() {\n"
+        "  const float xmin = 0.0f, xmax = 4.0f, dt = 1.0f / 120.0f;\n"
+        "  const float push = (xmax - xmin) / 3.0f;\n"
+        "  float wave = 0.0f;\n"
+        "  for (int k = 0; k < 400; ++k) {\n"
+        "    std::printf(\"%a\\n\", (double)waveGeneratorValue(push, wave));\n"
+        "    wave += dt;\n"
+        "  }\n}\n")
+    exe = tmp_path / "wave"
+    import subprocess
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                    "-I", os.path.join(ROOT, "position-based-fluids_amd", "csrc"),
+                    str(src), "-o", str(exe)], check=True)
+    got = [float.fromhex(v) for v in
+           subprocess.run([str(exe)], check=True, capture_output=True,
+                          text=True).stdout.split()]
+    f32 = np.float32
+    push = f32(f32(4.0) - f32(0.0)) / f32(3.0)
+    phase = float(f32(-np.pi / 2.0))
+
+    def sequence(ph):
+        wave, out = f32(0.0), []
+        for _ in range(400):
+            arg = 2.0 * np.pi * float(f32(1.0)) * float(wave) + ph
+            v = np.sin(arg) * float(push) / 2.0 + float(push / f32(2.0))
+            out.append(float(f32(v)))
+            wave = f32(wave + f32(1.0 / 120.0))
+        return out
+
+    expect = sequence(phase)
+    assert got == expect
+    # the exact-double phase gives other floats: the test tells them apart
+    assert sum(a != b for a, b in zip(sequence(-np.pi / 2.0), expect)) > 50
