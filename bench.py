@@ -6,6 +6,12 @@ iterations, FP32).  A step is one pass of the hot path: predict, cell sort,
 4 x (lambda, delta), XSPH + vorticity, position update, launched as one
 captured hipGraph.  Prints one JSON line.
 
+Roofline: algorithmic bytes are the compulsory streams of each pass counted
+once (324 + 60 I bytes per particle-step in PAPER physics, 244 + 60 I in REF,
+plus 16 bytes per grid cell; 36 bytes per particle for a delta pass, 24 for
+a lambda pass), over the measured time, against 8 TB/s.  The neighbour lists
+this build streams are reported beside it (`with_lists`), not in it.
+
   python bench.py [--steps K] [--warmup W] [--config 1] [--physics paper]
 """
 import argparse
@@ -24,10 +30,48 @@ import scenarios  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic (compulsory) HBM bytes per particle per launch of each kernel
-# class, FP32.  PAPER passes stream a neighbour list of L = ceil(n/11)
-# 16-byte groups (n neighbours, 11 bits each); "delta" is the mean over the
-# I launches (the last one also reads rho and x_n and writes v).
+# Algorithmic (compulsory) HBM bytes per particle of the step's passes: the
+# streams each pass must read and write once, neighbour re-reads counted as
+# 0 (float4 = 16 B; lambda, rho, key and index 4 B each; the cell table 16 B
+# per cell, written and cleared).  These are the figures `roofline` uses.
+PASS_BYTES = {
+    "predict_hash": 56.0,   # x, v in; x*, key, id out
+    "sort": 20.0,           # keys x2, id in; sorted id out
+    "reorder": 64.0,        # x, x* in; x, x* out
+    "cell_table": 4.0,      # sorted keys (+ 16 B per cell below)
+    "lambda": 24.0,         # x* in; lambda, rho out (per iteration)
+    "delta": 36.0,          # x*, lambda in; x* out (per iteration)
+    "velocity": 48.0,       # x, x* in; v out
+    "xsph_paper": 68.0,     # x*, v, rho in; v', omega out
+    "xsph_ref": 52.0,       # x*, v, rho in; v' out
+    "vorticity": 64.0,      # x*, omega, v' in; v out
+}
+
+
+def step_bytes_per_particle(iterations: int, physics: int) -> float:
+    """324 + 60 I bytes per particle (PAPER), 244 + 60 I (REF); the cell
+    table adds 16 bytes per cell."""
+    b = (PASS_BYTES["predict_hash"] + PASS_BYTES["sort"] + PASS_BYTES["reorder"]
+         + PASS_BYTES["cell_table"] + PASS_BYTES["velocity"]
+         + iterations * (PASS_BYTES["lambda"] + PASS_BYTES["delta"]))
+    if physics == pbf.PHYSICS_PAPER:
+        b += PASS_BYTES["xsph_paper"] + PASS_BYTES["vorticity"]
+    else:
+        b += PASS_BYTES["xsph_ref"]
+    return b
+
+
+def pass_bytes(cls: str) -> float:
+    """Algorithmic bytes per particle of one launch of a kernel class that
+    can be timed alone (the searching lambda pass, a lambda replay, a delta
+    pass)."""
+    return {"search": PASS_BYTES["lambda"], "lambda": PASS_BYTES["lambda"],
+            "delta": PASS_BYTES["delta"]}[cls]
+
+
+# The PAPER passes also stream the step's neighbour lists: L = 16 ceil(n/11)
+# bytes for n neighbours (11-bit entries), plus the count and the 9 row
+# starts.  Reported beside the roofline (`with_lists`), never in it.
 LIST_PER_GROUP = 11
 LIST_MAX = 99
 
@@ -40,51 +84,10 @@ def list_bytes(counts) -> float:
                                   16.0 * np.ceil(c / LIST_PER_GROUP), 0.0)))
 
 
-def algorithmic_bytes(cls: str, iterations: int, physics: int,
-                      lb: float = 80.0) -> float:
-    last_extra = 36.0 / iterations
-    if physics == pbf.PHYSICS_PAPER:
-        table = {
-            "predict_count": 56.0,      # x, v in; x*, key, rank out
-            "scatter": 24.0,
-            "reorder": 88.0,
-            # own record in; (x*, lambda), rho, count, 9 row starts, list out
-            "search": 76.0 + lb,
-            # x, count, 9 row starts, list in; (x, lambda), rho out
-            "lambda": 76.0 + lb,
-            # (x, lambda), count, 9 row starts, list in; x out (the last
-            # pass also reads rho and x_n and writes the 32-byte x, v record)
-            "delta": 72.0 + lb + last_extra,
-            # own (x, rho, v, m), count, row starts, list in; v', omega, (x, |w|) out
-            "xsph": 120.0 + lb,
-            # (x, |w|), omega, v', count, row starts, list, id in; x, v, id out
-            "vorticity": 128.0 + lb,
-        }
-    else:
-        table = {
-            "predict_count": 56.0, "scatter": 24.0, "reorder": 88.0,
-            "lambda": 24.0, "delta": 36.0 + last_extra, "xsph": 84.0,
-        }
-    return table.get(cls, 0.0)
-
-
-def step_bytes(n: int, cells: int, iterations: int, physics: int,
-               lb: float = 80.0) -> float:
-    """Compulsory bytes of one step: the sum of the per-launch figures above
-    plus 16 bytes per cell (count, scan, table)."""
-    per = sum(algorithmic_bytes(c, iterations, physics, lb)
-              for c in ("predict_count", "scatter", "reorder"))
-    if physics == pbf.PHYSICS_PAPER:
-        per += (algorithmic_bytes("search", iterations, physics, lb)
-                + (iterations - 1) * algorithmic_bytes("lambda", iterations, physics, lb)
-                + iterations * algorithmic_bytes("delta", iterations, physics, lb)
-                + algorithmic_bytes("xsph", iterations, physics, lb)
-                + algorithmic_bytes("vorticity", iterations, physics, lb))
-    else:
-        per += iterations * (algorithmic_bytes("lambda", iterations, physics)
-                             + algorithmic_bytes("delta", iterations, physics))
-        per += algorithmic_bytes("xsph", iterations, physics)
-    return n * per + 16.0 * cells
+def list_pass_bytes(cls: str, lb: float) -> float:
+    """Bytes per particle of one launch including the list, count and row
+    starts it streams (PAPER)."""
+    return {"search": 76.0 + lb, "lambda": 76.0 + lb, "delta": 72.0 + lb}[cls]
 
 
 def load_traffic(workload: str):
@@ -110,7 +113,27 @@ def kernel_class(name: str):
     return None
 
 
-def cpu_baseline(params, parts, steps: int, threads: int):
+def cpu_info() -> dict:
+    """Host core counts and CPU model of the machine the bench runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "usable": usable, "cpu_model": model}
+
+
+def cpu_rate(params, parts, steps: int, threads: int):
+    """The FP32 oracle restatement stepped `steps` times on `threads` host
+    threads (step loop only); returns (particle-steps/s, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pbf_oracle as orc
 
@@ -120,6 +143,42 @@ def cpu_baseline(params, parts, steps: int, threads: int):
     orc.run_f32(params, pos, vel, steps)
     dt = time.perf_counter() - t0
     return parts.shape[0] * steps / dt, dt
+
+
+def cpu_baseline(sc, physics_name: str, steps: int, one_core_steps: int,
+                 ref_steps: int) -> dict:
+    """The oracle's FP32 restatement timed on the host: the benchmark's own
+    workload on the host's core share (at most 16 threads: the share of one
+    GPU on these machines) and on 1 core, plus the reference kernels'
+    physics (REF) on the 32,768-particle dam break (c0)."""
+    info = cpu_info()
+    threads = max(1, min(16, info["usable"]))
+    params, parts = sc.params[0], sc.particles[0]
+    rate, secs = cpu_rate(params, parts, steps, threads)
+    one, one_secs = cpu_rate(params, parts, one_core_steps, 1)
+    c0 = scenarios.config(0, physics=pbf.PHYSICS_REF)
+    ref, ref_secs = cpu_rate(c0.params[0], c0.particles[0], ref_steps, threads)
+    ref1, ref1_secs = cpu_rate(c0.params[0], c0.particles[0],
+                               max(1, ref_steps // 10), 1)
+    return {
+        "value": rate,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle FP32 restatement, the same {sc.name} input and "
+                   f"{physics_name} physics, {steps} step(s), {secs:.1f} s on "
+                   f"{threads} threads"),
+        "nproc": info["nproc"],
+        "usable_cores": info["usable"],
+        "cpu_model": info["cpu_model"],
+        "one_core": {"value": one, "steps": one_core_steps,
+                     "seconds": one_secs},
+        "ref_c0": {"value": ref, "threads": threads, "steps": ref_steps,
+                   "seconds": ref_secs, "one_core_value": ref1,
+                   "one_core_steps": max(1, ref_steps // 10),
+                   "workload": "c0: 32,768-particle dam break, REF physics "
+                               "(the reference kernels' equations), FP32"},
+    }
 
 
 def _short(name: str) -> str:
@@ -264,16 +323,49 @@ def _run_to(params, parts, steps: int, timed: int, warmup: int):
     return ms, diag, pos
 
 
+def _state_at(params, parts, steps: int):
+    """Positions, velocities and GPU diagnostics after `steps` steps of a
+    fresh handle."""
+    with pbf.Simulation(params, parts) as s:
+        if steps:
+            s.step(steps)
+        pos, vel = s.dump()
+        diag = s.diagnostics()
+    return pos, vel, diag
+
+
+def _density_errors(rho, rho0) -> dict:
+    """Density error figures of pbf_diagnostics for FP64 densities: the
+    largest compression over all particles, and max / mean |rho/rho0 - 1|
+    over particles with a neighbour."""
+    e = rho / rho0 - 1.0
+    has = rho > 0.0
+    return {"max_compression": float(max(e.max(), 0.0)),
+            "max_abs_density_err": float(np.abs(e[has]).max()) if has.any() else 0.0,
+            "mean_abs_density_err": float(np.abs(e[has]).mean()) if has.any() else 0.0}
+
+
+def _vs_oracle(x, ref) -> dict:
+    """Position error of a GPU state against the FP64 oracle's."""
+    d = x[:, :3].astype(np.float64) - ref[:, :3]
+    dist = np.sqrt(np.sum(d * d, axis=1))
+    k = int(np.argmax(dist))
+    return {"rms": float(np.sqrt(np.mean(dist * dist))), "max": float(dist[k]),
+            "p99": float(np.percentile(dist, 99)), "p999": float(np.percentile(dist, 99.9)),
+            "worst_particle": k,
+            "worst_gpu_x": [float(v) for v in x[k, :3]],
+            "worst_fp64_x": [float(v) for v in ref[k, :3]]}
+
+
 def mixed_vs_plain(timed: int, oracle_threads: int, with_oracle: bool):
     """The 4,194,304-particle sphere dam (c4), PAPER physics: FP32 against
-    mixed precision (iteration 0 the FP32 neighbour search, the lambda
-    passes of iterations 1-2 over FP16 neighbour positions, iteration 3
-    FP32), both run
-    50 steps from the same input; the speed of each over `timed` graph
-    launches, the GPU FP64 density error after 50 steps, and the position
-    RMS between the two.  The FP64 oracle comparison runs on a 65,536-
-    particle replica of the same geometry (64 x 32 x 32 block, 6 x 2 x 2
-    box, seeded sphere) for 50 steps."""
+    mixed precision, both run 50 steps from the same input; the speed of
+    each over `timed` graph launches, the GPU FP64 density error after 50
+    steps, and the position RMS between the two.  With the oracle: the FP64
+    oracle runs the same 4,194,304 particles for 50 steps on the host, and
+    the FP32 and mixed runs are compared with it after 1, 10 and 50 steps
+    (position RMS, maximum, and where the largest error sits) together with
+    the FP64 density error of each state."""
     sc = scenarios.config(4)
     p32 = sc.params[0]
     pmx = p32.copy()
@@ -294,6 +386,8 @@ def mixed_vs_plain(timed: int, oracle_threads: int, with_oracle: bool):
         "speedup": ms32 / msmx,
         "fp32_max_density_err": d32["max_compression"],
         "mixed_max_density_err": dmx["max_compression"],
+        "fp32_max_abs_density_err": d32["max_abs_density_err"],
+        "mixed_max_abs_density_err": dmx["max_abs_density_err"],
         "fp32_mean_density_err": d32["mean_abs_density_err"],
         "mixed_mean_density_err": dmx["mean_abs_density_err"],
         "position_rms_mixed_vs_fp32": float(np.sqrt(np.mean(
@@ -305,26 +399,28 @@ def mixed_vs_plain(timed: int, oracle_threads: int, with_oracle: bool):
         import pbf_oracle as orc
 
         orc.set_threads(oracle_threads)
-        small = scenarios.sphere_dam((64, 32, 32), (6.0, 2.0, 2.0), seed=1005,
-                                     steps=50)
-        q32 = small.params[0]
-        qmx = q32.copy()
-        qmx.precision, qmx.fp32_iterations = pbf.PRECISION_MIXED, 1
-        sp = small.particles[0]
-        pos, vel = orc.state_from_particles(sp, np.float64)
-        t0 = time.perf_counter()
-        op, _ = orc.run_f64(q32, pos, vel, small.steps)
-        secs = time.perf_counter() - t0
-        rho = orc.density_f64(q32, op)
-        res = {"particles": small.n, "steps": small.steps,
-               "oracle_seconds": secs, "oracle_threads": oracle_threads,
-               "fp64_max_density_err": float(np.abs(rho / q32.rest_density - 1).max())}
-        for tag, q in (("fp32", q32), ("mixed", qmx)):
-            _, d, x = _run_to(q, sp, small.steps, 1, 1)
-            e = x[:, :3].astype(np.float64) - op[:, :3]
-            res[f"{tag}_position_rms_vs_fp64"] = float(np.sqrt(np.mean(np.sum(e * e, axis=1))))
-            res[f"{tag}_position_max_vs_fp64"] = float(np.abs(e).max())
-            res[f"{tag}_max_density_err"] = d["max_compression"]
+        pos, vel = orc.state_from_particles(parts, np.float64)
+        res = {"particles": n, "oracle_threads": oracle_threads, "steps": {}}
+        t_oracle = 0.0
+        done = 0
+        for k in (1, 10, sc.steps):
+            t0 = time.perf_counter()
+            pos, vel = orc.run_f64(p32, pos, vel, k - done)
+            t_oracle += time.perf_counter() - t0
+            done = k
+            at = {"fp64": _density_errors(orc.density_grid_f64(p32, pos),
+                                          p32.rest_density)}
+            for tag, q, x_end in (("fp32", p32, x32), ("mixed", pmx, xmx)):
+                x = x_end if k == sc.steps else _state_at(q, parts, k)[0]
+                at[tag] = _vs_oracle(x, pos)
+                at[tag].update(_density_errors(
+                    orc.density_grid_f64(p32, x.astype(np.float64)),
+                    p32.rest_density))
+            res["steps"][str(k)] = at
+            print(f"[bench] c4 vs FP64 oracle at step {k}: rms fp32 "
+                  f"{at['fp32']['rms']:.2e} mixed {at['mixed']['rms']:.2e} m",
+                  file=sys.stderr, flush=True)
+        res["oracle_seconds"] = t_oracle
         out["vs_fp64_oracle"] = res
     return out
 
@@ -338,7 +434,15 @@ def main():
     ap.add_argument("--physics", choices=["paper", "ref"], default="paper")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=60,
+                    help="steps of the CPU baseline on the host's core share")
+    ap.add_argument("--cpu-one-core-steps", type=int, default=3,
+                    help="steps of the CPU baseline on one core")
+    ap.add_argument("--cpu-ref-steps", type=int, default=100,
+                    help="steps of the REF-physics c0 CPU leg")
+    ap.add_argument("--horizon", type=int, default=200,
+                    help="steps of the fresh-handle run timed from step 0 "
+                         "(0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="steps of the 256-scene batch vs sequential "
@@ -416,14 +520,16 @@ def main():
     lb = list_bytes(sim.probe()["nbr_count"])
     sim.close()
 
-    total_prof = sum(v[0] for v in prof.values())
     dom_launches = prof[dom][1]
-    dom_bytes = algorithmic_bytes(dom, iters, physics, lb) * n
-    achieved = dom_bytes / (avg_ms / 1e3) / 1e9
-    workload = f"c{args.config}"
-    traffic = load_traffic(workload)
-
-    b_step = step_bytes(n, cells, iters, physics, lb)
+    total_prof = sum(v[0] for v in prof.values())
+    alg = pass_bytes(dom) * n
+    achieved = alg / (avg_ms / 1e3) / 1e9
+    traffic = (load_traffic(f"c{args.config}") or {}).get("bytes_per_launch", {}).get(dom)
+    step_b = step_bytes_per_particle(iters, physics)
+    # the cell table of the configuration's grid (x_n y_n z_n cells; the
+    # PAPER search bins x four times finer, which is this build's choice)
+    grid_cells = int(params.x_n * params.y_n * params.z_n)
+    b_step = step_b * n + 16.0 * grid_cells
     step_gbs = b_step * args.steps / (ms / 1e3) / 1e9
 
     out = {
@@ -456,13 +562,21 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (traffic or {}).get("bytes_per_launch", {}).get(dom),
+            "traffic": traffic,
+            "traffic_over_algorithmic": (traffic / alg) if traffic else None,
             "avg_launch_ms": avg_ms,
             "launches_per_step": dom_launches / args.profile_steps,
-            "algorithmic_bytes_per_launch": dom_bytes,
+            "algorithmic_bytes_per_particle": pass_bytes(dom),
+            "algorithmic_bytes_per_launch": alg,
+            "with_lists": {
+                "bytes_per_particle": list_pass_bytes(dom, lb),
+                "list_bytes_per_particle": lb,
+                "achieved_GBps": list_pass_bytes(dom, lb) * n / (avg_ms / 1e3) / 1e9,
+                "frac": list_pass_bytes(dom, lb) * n / (avg_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            },
         },
         "step_roofline": {
-            "list_bytes_per_particle": lb,
+            "bytes_per_particle": step_b + 16.0 * grid_cells / n,
             "bytes_per_step": b_step,
             "achieved_GBps": step_gbs,
             "frac": step_gbs / HBM_PEAK_GBS,
@@ -471,30 +585,41 @@ def main():
         "profile_overhead": total_prof / args.profile_steps / (ms / args.steps),
         "accuracy": {
             "max_compression": diag["max_compression"],
+            "max_abs_density_err": diag["max_abs_density_err"],
             "mean_abs_density_err": diag["mean_abs_density_err"],
+            "at_step": args.warmup + args.steps,
         },
     }
+    if args.horizon > 0:
+        # the configuration's own horizon, timed from step 0 of a fresh
+        # handle (the autotuner's trial steps restore the state first)
+        with pbf.Simulation(params, parts) as h:
+            hms = h.time_steps(args.horizon)
+            hd = h.diagnostics()
+        out["horizon"] = {
+            "steps": args.horizon,
+            "ms_per_step": hms / args.horizon,
+            "particle_steps_per_s": n * args.horizon / (hms / 1e3),
+            "step_roofline_frac": b_step * args.horizon / (hms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "max_compression": hd["max_compression"],
+            "mean_abs_density_err": hd["mean_abs_density_err"],
+        }
+        print(f"[bench] steps 0-{args.horizon} of a fresh handle: "
+              f"{hms / args.horizon:.4f} ms/step", file=sys.stderr, flush=True)
     if args.batch_steps > 0:
         print(f"[bench] 256-scene batch vs sequential, {args.batch_steps} steps",
               file=sys.stderr, flush=True)
         out["batch"] = batch_vs_sequential(physics, args.batch_steps)
+    threads = max(1, min(16, cpu_info()["usable"]))
     if args.mixed_steps > 0 and physics == pbf.PHYSICS_PAPER:
         print(f"[bench] c4 mixed precision vs FP32, {args.mixed_steps} timed steps",
               file=sys.stderr, flush=True)
-        out["mixed"] = mixed_vs_plain(args.mixed_steps, min(16, os.cpu_count() or 1),
-                                      not args.no_cpu)
+        out["mixed"] = mixed_vs_plain(args.mixed_steps, threads, not args.no_cpu)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cps, secs = cpu_baseline(params, parts, args.cpu_steps, threads)
-        out["cpu_baseline"] = {
-            "value": cps,
-            "unit": "particle-steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": (f"oracle FP32 restatement, same {sc.name} input and "
-                       f"{args.physics} physics, {args.cpu_steps} step(s), "
-                       f"{secs:.1f} s on {threads} threads"),
-        }
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(sc, args.physics, args.cpu_steps,
+                                           args.cpu_one_core_steps,
+                                           args.cpu_ref_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -707,6 +707,34 @@ int orc_density_f64(const pbf_params *p, int n, const double *pos4,
   return 0;
 }
 
+// The same density through the cell lists of the parameters' grid (any n):
+// the 27-cell stencil holds every j within h when the cells are at least h
+// long (REF grids: cell length = h; PAPER needs y, z cells >= h and x cells
+// = h).
+int orc_density_grid_f64(const pbf_params *p, int n, const double *pos4,
+                         double *rho) {
+  if (!params_ok(p, n) || !rho) return 1;
+  State<double> s;
+  init_state(s, *p, n);
+  std::memcpy(s.pred.data(), pos4, sizeof(double) * 4 * (size_t)n);
+  build_lists(s);
+  const Consts<double> &k = s.k;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    double acc = 0;
+    for_candidates(s, i, [&](int j) {
+      double r[3];
+      const double r2 = dist2(&s.pred[4 * (size_t)i], &s.pred[4 * (size_t)j], r);
+      if (r2 > 0 && r2 < k.h2) {
+        const double t = k.h2 - r2;
+        acc += k.poly6 * t * t * t;
+      }
+    });
+    rho[i] = acc;
+  }
+  return 0;
+}
+
 // PAPER vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15) at the
 // given positions and velocities (FP64; lists built from the positions).
 int orc_vorticity_f64(const pbf_params *p, int n, const double *pos4,

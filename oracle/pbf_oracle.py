@@ -42,9 +42,10 @@ def load():
     lib.orc_density_f64.argtypes = [vp, ctypes.c_int, vp, vp]
     lib.orc_vorticity_f64.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     lib.orc_vorticity_force_f64.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+    lib.orc_density_grid_f64.argtypes = [vp, ctypes.c_int, vp, vp]
     for f in ("orc_constants", "orc_run_f32", "orc_run_f64", "orc_probe_f32",
               "orc_brute_neighbors_f32", "orc_density_f64", "orc_vorticity_f64",
-              "orc_vorticity_force_f64"):
+              "orc_vorticity_force_f64", "orc_density_grid_f64"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -142,6 +143,15 @@ def density_f64(params, pos4) -> np.ndarray:
     pos = np.ascontiguousarray(pos4, np.float64)
     out = np.zeros(pos.shape[0], np.float64)
     assert load().orc_density_f64(_pp(params), pos.shape[0], _p(pos), _p(out)) == 0
+    return out
+
+
+def density_grid_f64(params, pos4) -> np.ndarray:
+    """FP64 density at the given positions through the cell lists of the
+    parameters' grid (large n)."""
+    pos = np.ascontiguousarray(pos4, np.float64)
+    out = np.zeros(pos.shape[0], np.float64)
+    assert load().orc_density_grid_f64(_pp(params), pos.shape[0], _p(pos), _p(out)) == 0
     return out
 
 

@@ -281,3 +281,22 @@ def test_wave_generator_sequence_matches_reference_formula(tmp_path):
     assert got == expect
     # the exact-double phase gives other floats: the test tells them apart
     assert sum(a != b for a, b in zip(sequence(-np.pi / 2.0), expect)) > 50
+
+
+def test_bench_algorithmic_bytes():
+    """The roofline's byte model: 324 + 60 I bytes per particle-step (PAPER),
+    244 + 60 I (REF), 16 per grid cell; 570.9 B at the 1M-particle dam break
+    (432,000 cells); a delta pass 36 B, a lambda pass 24 B per particle."""
+    import bench
+    assert bench.step_bytes_per_particle(4, pbf.PHYSICS_PAPER) == 564.0
+    assert bench.step_bytes_per_particle(4, pbf.PHYSICS_REF) == 484.0
+    assert bench.step_bytes_per_particle(1, pbf.PHYSICS_PAPER) == 384.0
+    c1 = scenarios.config(1)
+    p = c1.params[0]
+    cells = p.x_n * p.y_n * p.z_n
+    assert cells == 432000
+    b = bench.step_bytes_per_particle(4, pbf.PHYSICS_PAPER) + 16.0 * cells / c1.n
+    assert b == pytest.approx(570.912)
+    assert bench.pass_bytes("delta") == 36.0 and bench.pass_bytes("lambda") == 24.0
+    info = bench.cpu_info()
+    assert info["nproc"] >= 1 and info["usable"] >= 1 and info["cpu_model"]

@@ -585,3 +585,17 @@ def test_sphere_projection_runs_before_the_wall_clamp():
     x, _ = _one_particle_step([1.0, 0.12, 1.0], (1.0, 0.2, 1.0, 0.05))
     assert x[1] == 0.1
     assert x[0] == 1.0 and x[2] == 1.0
+
+
+def test_grid_density_equals_brute_force():
+    """The cell-list FP64 density (used at sizes where the brute-force sum is
+    out of reach) against the O(N^2) sum on a settled dam."""
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=4)
+    p = sc.params[0]
+    pos, vel = orc.state_from_particles(sc.particles[0], np.float64)
+    x, _ = orc.run_f64(p, pos, vel, 20)
+    a = orc.density_f64(p, x)
+    b = orc.density_grid_f64(p, x)
+    assert np.allclose(a, b, rtol=1e-13, atol=1e-9)
+    assert a.max() > 6000.0
