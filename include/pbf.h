@@ -47,10 +47,14 @@ extern "C" {
 
 /* ---- precision --------------------------------------------------------- */
 /* PBF_PRECISION_MIXED (PAPER only): the lambda passes of iterations
- * 1 .. iterations - fp32_iterations - 1 gather FP16 bin-relative neighbour
- * positions (8-byte records; kernels evaluated and accumulated in FP32).
- * Iteration 0 is the FP32 neighbour search, whose lists every pass of the
- * step uses, and the last fp32_iterations iterations are full FP32. */
+ * 1 .. iterations - fp32_iterations - 1 and the XSPH and vorticity passes
+ * gather 16-bit neighbour records: positions as 16-bit fixed point in units
+ * of h / 4096 kept modulo 2^16, |omega|, density and velocity as FP16
+ * (8-byte records; 16 bytes with the velocity).  Kernels are evaluated and
+ * accumulated in FP32.  Iteration 0's lambda pass is the FP32 neighbour
+ * search, whose lists every pass of the step uses; every delta pass gathers
+ * FP32 records and updates FP32 positions, and the last fp32_iterations
+ * iterations are full FP32. */
 #define PBF_PRECISION_FP32 0
 #define PBF_PRECISION_MIXED 1
 
@@ -237,8 +241,8 @@ int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms);
  * own size, chosen by timing 64, 128 and 256 on the first step (cached per
  * shape, and in the file named by the PBF_TUNE_CACHE environment variable);
  * otherwise all use block_size (default 128).  cfg[1] = particles per
- * work-item (1), cfg[2] = sort passes (1: one counting pass), cfg[3] = FP16
- * lambda passes per step. */
+ * work-item (1), cfg[2] = sort passes (1: one counting pass), cfg[3] =
+ * iterations that gather 16-bit records (mixed precision). */
 int pbf_get_config(const pbf_sim *sim, int32_t cfg[4]);
 
 #ifdef __cplusplus

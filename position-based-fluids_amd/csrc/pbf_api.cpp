@@ -160,6 +160,8 @@ pbf::Scene make_scene(const pbf_params &p) {
     s.rho0 = p.rest_density;
     const double reach = h * (1.0 + (double)pbf::kReachMargin);
     s.reach2 = (float)(reach * reach);
+    s.q_scale = (float)((double)(1 << pbf::kQBits) / h);
+    s.q_unit = (float)(h / (double)(1 << pbf::kQBits));
   }
   s.inv_rho0 = 1.0f / s.rho0;
   s.eps = p.relaxation;
@@ -337,15 +339,12 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   sh.scan_blocks = (int)((cells + pbf::kScanTile) / pbf::kScanTile);
   sh.physics = p0.physics == PBF_PHYSICS_REF ? pbf::kRef : pbf::kPaper;
   sh.iterations = p0.iterations;
-  // iteration 0 searches in FP32; iterations 1 .. I - F - 1 are FP16
+  // the lambda passes of iterations 1 .. I - F - 1, XSPH and the vorticity
+  // pass gather 16-bit records; iteration 0's search, every delta pass and
+  // the last F iterations are FP32
   sh.fp16_iterations = p0.precision == PBF_PRECISION_MIXED
-                           ? std::max(p0.iterations - p0.fp32_iterations - 1, 0)
+                           ? std::max(p0.iterations - p0.fp32_iterations, 0)
                            : 0;
-  if (sh.fp16_iterations > 0)
-    for (const pbf::Scene &sc : S.scenes)
-      if (sc.nc[0] > 65535)  // the FP16 records hold the fine x index in 16 bits
-        invalid("mixed precision supports at most 65535 fine x cells (" +
-                std::to_string(sc.nc[0] / pbf::kXSub) + " x cells given)");
   bool vort = false;
   for (uint32_t s = 0; s < nscenes; ++s)
     vort = vort || params[s].vorticity_eps != 0.0f;
@@ -388,8 +387,8 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     d.om = S.alloc<float4>(P);
   }
   if (sh.fp16_iterations > 0) {
-    d.xh = S.alloc<uint2>(P);
-    d.kc = S.alloc<int>(P);
+    d.qa = S.alloc<uint2>(P);
+    d.qv = S.alloc<uint4>(P);
   }
   if (sh.physics == pbf::kPaper) {
     d.nl = S.alloc<uint4>(P * (size_t)pbf::kMaxGroups);

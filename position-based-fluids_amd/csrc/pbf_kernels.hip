@@ -70,27 +70,51 @@ __device__ __forceinline__ bool in_reach(float r2, float h2) {
   return __float_as_uint(r2) - 1u < __float_as_uint(h2) - 1u;
 }
 
-// Mixed precision: a position as FP16 offsets from the origin of its bin
-// (fine x cell, y/z cell), packed (x | y << 16, z | fx << 16) in 8 bytes
-// with fx the bin's fine x index (< 65536: checked at set-up).
-__device__ __forceinline__ uint2 pack_rel(const Scene &sc, float3 x, int c) {
-  const int local = c - sc.cell_base;
-  const int fx = local % sc.nc[0], t = local / sc.nc[0];
-  const int cy = t % sc.nc[1], cz = t / sc.nc[1];
-  const float ox = sc.bmin[0] + fx * sc.cell_len[0];
-  const float oy = sc.bmin[1] + cy * sc.cell_len[1];
-  const float oz = sc.bmin[2] + cz * sc.cell_len[2];
-  const unsigned hx = __half_as_ushort(__float2half_rn(x.x - ox));
-  const unsigned hy = __half_as_ushort(__float2half_rn(x.y - oy));
-  const unsigned hz = __half_as_ushort(__float2half_rn(x.z - oz));
-  return make_uint2(hx | (hy << 16), hz | ((unsigned)fx << 16));
+// Mixed precision records.  A position is 16-bit fixed point in units of
+// q_unit = h / 2^kQBits from the scene's lower corner, each coordinate kept
+// modulo 2^16, packed (x | y << 16, z | payload << 16); payloads and the
+// velocities of the XSPH record are FP16.
+__device__ __forceinline__ unsigned q16(float v, float lo, float scale) {
+  return (unsigned)__float2int_rn((v - lo) * scale) & 0xffffu;
 }
 
-__device__ __forceinline__ float half_lo(unsigned w) {
+__device__ __forceinline__ unsigned f16_bits(float v) {
+  return (unsigned)__half_as_ushort(__float2half_rn(v));
+}
+
+__device__ __forceinline__ float f16_lo(unsigned w) {
   return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu)));
 }
-__device__ __forceinline__ float half_hi(unsigned w) {
+
+__device__ __forceinline__ float f16_hi(unsigned w) {
   return __half2float(__ushort_as_half((unsigned short)(w >> 16)));
+}
+
+__device__ __forceinline__ uint2 pack_q(const Scene &sc, float3 x,
+                                        unsigned payload) {
+  const unsigned qx = q16(x.x, sc.bmin[0], sc.q_scale);
+  const unsigned qy = q16(x.y, sc.bmin[1], sc.q_scale);
+  const unsigned qz = q16(x.z, sc.bmin[2], sc.q_scale);
+  return make_uint2(qx | (qy << 16), qz | (payload << 16));
+}
+
+// XSPH record: position, rho / rho0 and v.
+__device__ __forceinline__ uint4 pack_qv(const Scene &sc, float3 x, float rho,
+                                         float3 v) {
+  const uint2 q = pack_q(sc, x, f16_bits(rho * sc.inv_rho0));
+  return make_uint4(q.x, q.y, f16_bits(v.x) | (f16_bits(v.y) << 16),
+                    f16_bits(v.z));
+}
+
+// a - b of two packed positions in units of q_unit: each coordinate
+// difference taken modulo 2^16 as a signed 16-bit value (exact for the
+// quantised points while they are less than 2^15 units, 8 h, apart).
+__device__ __forceinline__ float3 q_diff(unsigned ax, unsigned ay,
+                                         unsigned bx, unsigned by) {
+  const int dx = (int)(short)(unsigned short)(ax - bx);
+  const int dy = (int)(short)(unsigned short)((ax >> 16) - (bx >> 16));
+  const int dz = (int)(short)(unsigned short)(ay - by);
+  return make_float3((float)dx, (float)dy, (float)dz);
 }
 
 // Gathers through a buffer resource: a 32-bit byte offset per lane instead
@@ -625,7 +649,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     const int *__restrict__ key, const int *__restrict__ cell_start,
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
     int *__restrict__ pid_s, float4 *__restrict__ xa,
-    float4 *__restrict__ xold, int *__restrict__ kc) {
+    float4 *__restrict__ xold) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int t = lb * blockDim.x + threadIdx.x;
@@ -645,7 +669,6 @@ __global__ __launch_bounds__(256) void k_reorder(
   pid_s[f] = p;
   xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
   xold[f] = make_float4(x.x, x.y, x.z, pu.w);
-  if (kc) kc[f] = c;  // mixed precision: the bin of each sorted slot
 }
 
 // ---------------------------------------------------------------------------
@@ -1036,6 +1059,21 @@ struct LambdaSums {
     B[i] = make_float4(xi.x, xi.y, xi.z, -(dens * sc.inv_rho0 - 1.0f) / denom);
     rho[i] = dens;
   }
+  // store() for sums formed in units of q_unit (h = 2^kQBits, as the
+  // mixed-precision passes form them): (h^2-r^2)^3 scales as q_unit^6, the
+  // gradient terms (h-r)^4 and ((h-r)^2 r/|r|)^2 as q_unit^4.
+  __device__ __forceinline__ void store_q(const Scene &sc, int i, float3 xi,
+                                          float4 *__restrict__ B,
+                                          float *__restrict__ rho) const {
+    const float u2 = sc.q_unit * sc.q_unit, u4 = u2 * u2;
+    const float dens = sc.poly6 * (u4 * u2) * wsum;
+    const float sp2 = sc.spiky * sc.spiky;
+    const float denom = sp2 * u4 * (gx * gx + gy * gy + gz * gz + g2) *
+                            (sc.inv_rho0 * sc.inv_rho0) +
+                        sc.eps;
+    B[i] = make_float4(xi.x, xi.y, xi.z, -(dens * sc.inv_rho0 - 1.0f) / denom);
+    rho[i] = dens;
+  }
 };
 
 // Iteration 0: searches the rows of x*_i, records N_i, and forms lambda.
@@ -1110,62 +1148,55 @@ __global__ __launch_bounds__(256) void k_lambda_list(
   ls.store(sc, i, xi, B, rho);
 }
 
-// Mixed precision, iterations >= 1: k_lambda_list over FP16 bin-relative
-// neighbour positions (XH, written by the delta pass before).  The entry's
-// row gives the y/z bin corner (rows of x*_i), the record the fine x index.
-__global__ __launch_bounds__(256) void k_lambda_list_h(
+// The support radius and its square in units of q_unit (exact).
+constexpr float kQH = (float)(1 << kQBits);
+constexpr float kQH2 = kQH * kQH;
+
+// Mixed precision, iterations 1 .. m - 1: k_lambda_list over 16-bit
+// fixed-point neighbour positions (QA, written by the delta pass before),
+// the sums formed in units of q_unit.
+__global__ __launch_bounds__(256) void k_lambda_list_q(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ XC,
-    const uint2 *__restrict__ XH, const int *__restrict__ cs,
+    const uint2 *__restrict__ QA, const int *__restrict__ cs,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
-    const int *__restrict__ rbase, int stride,
-    float4 *__restrict__ B, float *__restrict__ rho) {
+    const int *__restrict__ rbase, int stride, float4 *__restrict__ B,
+    float *__restrict__ rho) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 s4 = XS[i], x4 = XC[i];
-  const float3 xs = make_float3(s4.x, s4.y, s4.z);
+  const float4 x4 = XC[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  int cx, cy, cz;
-  cell_coords<kPaper>(sc, xs.x, xs.y, xs.z, cx, cy, cz);
-  // x_i relative to the corner of the lowest stencil row (dy = dz = 0)
-  const float qx = xi.x - sc.bmin[0], wx = sc.cell_len[0];
-  const float qy0 = xi.y - (sc.bmin[1] + (cy - 1) * sc.cell_len[1]);
-  const float qz0 = xi.z - (sc.bmin[2] + (cz - 1) * sc.cell_len[2]);
+  // the own record, quantised as the neighbours' are (the padding entries,
+  // which name i itself, then give r = 0 exactly)
+  const uint2 qi = QA[i];
   LambdaSums ls;
-  const float h = sc.h, h2 = sc.h2;
   const Gather<float4> xsg(XS, stride);
-  const Gather<uint2> xhg(XH, stride);
-  for_neighbours(sc, cs, xsg, xhg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int j, int row, uint2 w) {
-    const int dz = (row * 11) >> 5;  // row / 3 for row < 9
-    const int dy = row - 3 * dz;
-    const float ox = (float)(w.y >> 16) * wx;
-    const float rx = (qx - ox) - half_lo(w.x);
-    const float ry = (qy0 - (float)dy * sc.cell_len[1]) - half_hi(w.x);
-    const float rz = (qz0 - (float)dz * sc.cell_len[2]) - half_lo(w.y);
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    if (j != i && in_reach(r2, h2)) ls.add(h, h2, rx, ry, rz, r2);
+  const Gather<uint2> qag(QA, stride);
+  for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
+                 [&](int, int, uint2 qj) {
+    const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
+    const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    ls.add_if(in_reach(r2, kQH2), kQH, kQH2, r.x, r.y, r.z, r2);
   });
-  ls.store(sc, i, xi, B, rho);
+  ls.store_q(sc, i, xi, B, rho);
 }
 
 // eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST writes
 // XV = ((x, rho), (v, m)) instead, v = (x - x_n)/dt, for the XSPH pass.
-// WRITE_H (mixed precision) also writes the FP16 bin-relative copy for the
-// next FP16 lambda pass.
-template <bool LAST, bool WRITE_H, bool N4>
+// WRITE_QA / WRITE_QV (mixed precision) also write the 16-bit records of
+// the next lambda pass / of the XSPH pass.
+template <bool LAST, bool WRITE_QA, bool N4, bool WRITE_QV>
 __global__ __launch_bounds__(256) void k_delta_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ B,
     const int *__restrict__ cs, const float *__restrict__ rho,
     const float4 *__restrict__ xold, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt,
-    const int *__restrict__ rbase, int stride, const int *__restrict__ kc,
-    float4 *__restrict__ XC, XVRec *__restrict__ XV,
-    uint2 *__restrict__ XH) {
+    const int *__restrict__ ncnt, const int *__restrict__ rbase, int stride,
+    float4 *__restrict__ XC, XVRec *__restrict__ XV, uint2 *__restrict__ QA,
+    uint4 *__restrict__ QV) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -1208,24 +1239,92 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   const float c = -sc.spiky * sc.inv_rho0;
   const float3 dp = make_float3(ax * c, ay * c, az * c);
   const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
-  if constexpr (WRITE_H) XH[i] = pack_rel(sc, xn, kc[i]);
+  if constexpr (WRITE_QA) QA[i] = pack_q(sc, xn, 0u);
   if constexpr (LAST) {
     const float4 xo = xold[i];
     const float3 v = velocity_exact(sc, xn, xo);
     XV[i] = XVRec{make_float4(xn.x, xn.y, xn.z, rho[i]),
                   make_float4(v.x, v.y, v.z, xo.w)};
+    if constexpr (WRITE_QV) QV[i] = pack_qv(sc, xn, rho[i], v);
   } else {
     XC[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
   }
 }
 
+// The XSPH sum over N_i: (v_j - v_i) W_ij / rho_j, a neighbour with no
+// density (no pair within h in the last lambda pass) carrying no weight;
+// with OMEGA also omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15), spiky
+// applied by the caller.  QIN: neighbours from the 16-bit records QV, the
+// sums formed in units of q_unit (scale: s by q_unit^6 / rho0, since the
+// records hold rho / rho0; om by q_unit^2).
+template <bool QIN, bool OMEGA>
+__device__ __forceinline__ void xsph_sums(
+    const Scene &sc, const int *__restrict__ cs, const Gather<float4> &xsg,
+    const XVRec *__restrict__ XV, const uint4 *__restrict__ QV,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride, int i, int *rb, float3 xi,
+    float4 vi, float3 &s, float3 &om) {
+  const float h = QIN ? kQH : sc.h, h2 = QIN ? kQH2 : sc.h2;
+  auto term = [&](float rx, float ry, float rz, float rho_j, float3 vj) {
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    // branch-free: the terms of an out-of-reach entry are selected to 0
+    const bool in = in_reach(r2, h2);
+    const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
+    const float t = h2 - r2;
+    const float w =
+        in && rho_j > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(rho_j) : 0.0f;
+    s.x += vx * w;
+    s.y += vy * w;
+    s.z += vz * w;
+    if constexpr (OMEGA) {
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float gs = in ? hr * hr * inv : 0.0f;  // grad_j W_ij = spiky gs r
+      const float gx = rx * gs, gy = ry * gs, gz = rz * gs;
+      om.x += vy * gz - vz * gy;
+      om.y += vz * gx - vx * gz;
+      om.z += vx * gy - vy * gx;
+    }
+  };
+  if constexpr (QIN) {
+    const uint4 qi = QV[i];
+    const Gather<uint4> qvg(QV, stride);
+    for_neighbours(sc, cs, xsg, qvg, nl, ncnt, rbase, stride, i, rb,
+                   [&](int, int, uint4 qj) {
+      const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
+      term(r.x, r.y, r.z, f16_hi(qj.y),
+           make_float3(f16_lo(qj.z), f16_hi(qj.z), f16_lo(qj.w)));
+    });
+  } else {
+    const Gather<XVRec> xvg(XV, stride);
+    for_neighbours(sc, cs, xsg, xvg, nl, ncnt, rbase, stride, i, rb,
+                   [&](int, int, XVRec nb) {
+      term(xi.x - nb.x.x, xi.y - nb.x.y, xi.z - nb.x.z, nb.x.w,
+           make_float3(nb.v.x, nb.v.y, nb.v.z));
+    });
+  }
+}
+
+// Factor of the XSPH sum: c poly6 (FP32 sums), c poly6 q_unit^6 / rho0
+// (QIN sums).
+template <bool QIN>
+__device__ __forceinline__ float xsph_factor(const Scene &sc) {
+  if constexpr (QIN) {
+    const float u2 = sc.q_unit * sc.q_unit;
+    return sc.c_xsph * sc.poly6 * (u2 * u2 * u2) * sc.inv_rho0;
+  } else {
+    return sc.c_xsph * sc.poly6;
+  }
+}
+
 // XSPH over N_i with 1/rho_j weights fused with update_positions.cl (PAPER
 // without vorticity).
+template <bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
-    const int *__restrict__ cs, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt,
+    const uint4 *__restrict__ QV, const int *__restrict__ cs,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride,
     const int *__restrict__ pid_s, float4 *__restrict__ pos,
     float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err,
@@ -1238,44 +1337,35 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
   const XVRec own = XV[i];
   const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
   const float4 vi = own.v;
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  const float h2 = sc.h2;
+  float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
   const Gather<float4> xsg(XS, stride);
-  const Gather<XVRec> xvg(XV, stride);
-  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, XVRec nb) {
-    const float4 xj = nb.x, vj = nb.v;
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float t = h2 - r2;
-    // a neighbour with no density (no pair within h in the last lambda
-    // pass) carries no weight
-    const float w = in_reach(r2, h2) && xj.w > 0.0f
-                        ? t * t * t * __builtin_amdgcn_rcpf(xj.w)
-                        : 0.0f;
-    sx += (vj.x - vi.x) * w;
-    sy += (vj.y - vi.y) * w;
-    sz += (vj.z - vi.z) * w;
-  });
-  const float c = sc.c_xsph * sc.poly6;
+  xsph_sums<QIN, false>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
+                        rb_lds, xi, vi, s, om);
+  const float c = xsph_factor<QIN>(sc);
   const float4 v =
-      make_float4(vi.x + c * sx, vi.y + c * sy, vi.z + c * sz, vi.w);
+      make_float4(vi.x + c * s.x, vi.y + c * s.y, vi.z + c * s.z, vi.w);
   const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
   write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
   predict_count_slot<kPaper>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len),
                              v, 1, ns.pred_u, ns.key, ns.rank, ns.count);
 }
 
+// |omega| as the FP16 payload of the vorticity pass's records (scaled into
+// FP16's range).
+constexpr float kOmegaScale = 1.0f / 256.0f;
+
 // XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15) over
-// N_i; writes B = (x, |omega|) for the vorticity pass.
+// N_i; writes B = (x, |omega|) for the vorticity pass, or (QIN) its 16-bit
+// records QA.
+template <bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
-    const int *__restrict__ cs, const uint4 *__restrict__ nl,
-    const int *__restrict__ ncnt,
-    const int *__restrict__ rbase, int stride,
-    float4 *__restrict__ VX, float4 *__restrict__ OM,
-    float4 *__restrict__ B) {
+    const uint4 *__restrict__ QV, const int *__restrict__ cs,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride, float4 *__restrict__ VX,
+    float4 *__restrict__ OM, float4 *__restrict__ B,
+    uint2 *__restrict__ QA) {
   extern __shared__ int rb_lds[];
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -1284,48 +1374,32 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
   const XVRec own = XV[i];
   const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
   const float4 vi = own.v;
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  const float h = sc.h, h2 = sc.h2;
+  float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
   const Gather<float4> xsg(XS, stride);
-  const Gather<XVRec> xvg(XV, stride);
-  for_neighbours(sc, cs, xsg, xvg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, XVRec nb) {
-    const float4 xj = nb.x, vj = nb.v;
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    // branch-free: the terms of an out-of-reach entry are selected to 0
-    const bool in = in_reach(r2, h2);
-    const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
-    const float t = h2 - r2;
-    const float w =
-        in && xj.w > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(xj.w) : 0.0f;
-    sx += vx * w;
-    sy += vy * w;
-    sz += vz * w;
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float hr = h - r2 * inv;
-    const float gs = in ? hr * hr * inv : 0.0f;  // grad_j W_ij = spiky gs r
-    const float gx = rx * gs, gy = ry * gs, gz = rz * gs;
-    ox += vy * gz - vz * gy;
-    oy += vz * gx - vx * gz;
-    oz += vx * gy - vy * gx;
-  });
-  const float cx = sc.c_xsph * sc.poly6;
-  VX[i] = make_float4(vi.x + cx * sx, vi.y + cx * sy, vi.z + cx * sz, vi.w);
-  ox *= sc.spiky;
-  oy *= sc.spiky;
-  oz *= sc.spiky;
+  xsph_sums<QIN, true>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
+                       rb_lds, xi, vi, s, om);
+  const float cx = xsph_factor<QIN>(sc);
+  VX[i] = make_float4(vi.x + cx * s.x, vi.y + cx * s.y, vi.z + cx * s.z, vi.w);
+  const float cw = QIN ? sc.spiky * (sc.q_unit * sc.q_unit) : sc.spiky;
+  const float ox = om.x * cw, oy = om.y * cw, oz = om.z * cw;
   const float wl = sqrtf(ox * ox + oy * oy + oz * oz);
   OM[i] = make_float4(ox, oy, oz, wl);
-  B[i] = make_float4(xi.x, xi.y, xi.z, wl);
+  if constexpr (QIN)
+    QA[i] = pack_q(sc, xi, f16_bits(wl * kOmegaScale));
+  else
+    B[i] = make_float4(xi.x, xi.y, xi.z, wl);
 }
 
 // eta = sum (|w_j| - |w_i|) grad_i W_ij over N_i, N = eta/|eta|,
 // v += dt eps (N x w_i) (eq. 16), then update_positions.cl; reads
-// B = (x, |omega|).
+// B = (x, |omega|), or (QIN) the 16-bit records QA (eta formed in units of
+// q_unit and of the FP16 scale: only its direction is used) and the own
+// position from XV.
+template <bool QIN>
 __global__ __launch_bounds__(256) void k_vort_final(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ B,
+    const XVRec *__restrict__ XV, const uint2 *__restrict__ QA,
     const int *__restrict__ cs, const float4 *__restrict__ VX,
     const float4 *__restrict__ OM, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt,
@@ -1337,24 +1411,38 @@ __global__ __launch_bounds__(256) void k_vort_final(
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int i = lb * blockDim.x + threadIdx.x;
   if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = B[i];
+  const float4 x4 = QIN ? XV[i].x : B[i];
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   const float4 wi = OM[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  const float h = sc.h, h2 = sc.h2;
-  const Gather<float4> xsg(XS, stride), bg(B, stride);
-  for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, float4 xj) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+  const float h = QIN ? kQH : sc.h, h2 = QIN ? kQH2 : sc.h2;
+  auto term = [&](float rx, float ry, float rz, float dw) {
     const float r2 = rx * rx + ry * ry + rz * rz;
     const float inv = __builtin_amdgcn_rsqf(r2);
     const float hr = h - r2 * inv;
-    const float gs =
-        in_reach(r2, h2) ? (xj.w - wi.w) * hr * hr * inv : 0.0f;
+    const float gs = in_reach(r2, h2) ? dw * hr * hr * inv : 0.0f;
     ex += rx * gs;
     ey += ry * gs;
     ez += rz * gs;
-  });
+  };
+  const Gather<float4> xsg(XS, stride);
+  if constexpr (QIN) {
+    // |w_i| rounded as the neighbours' are, so that equal magnitudes cancel
+    const uint2 qi = QA[i];
+    const float wq = f16_hi(qi.y);
+    const Gather<uint2> qag(QA, stride);
+    for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
+                   [&](int, int, uint2 qj) {
+      const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
+      term(r.x, r.y, r.z, f16_hi(qj.y) - wq);
+    });
+  } else {
+    const Gather<float4> bg(B, stride);
+    for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
+                   [&](int, int, float4 xj) {
+      term(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, xj.w - wi.w);
+    });
+  }
   float4 v = VX[i];
   const float e2 = ex * ex + ey * ey + ez * ez;
   if (e2 > 0.0f) {
@@ -1577,7 +1665,7 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
              d.err);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
              d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
-             d.xa, d.xold, sh.fp16_iterations > 0 ? d.kc : nullptr);
+             d.xa, d.xold);
 }
 
 static NextSort next_sort(const DevBuffers &d) {
@@ -1606,17 +1694,22 @@ static void step_ref(const DevBuffers &d, Launcher &L) {
 }
 
 // A delta pass, at the s_corr exponent instance of the step's scenes.
-template <bool LAST, bool WRITE_H>
+template <bool LAST, bool WRITE_QA, bool WRITE_QV>
 static void delta_pass(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
   if (L.sh.scorr_n4)
-    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_H, true>), 9,
-               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
-               d.xold, d.nl, d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
+    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_QA, true, WRITE_QV>),
+               9, d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+               d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
   else
-    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_H, false>), 9,
-               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
-               d.xold, d.nl, d.ncnt, d.rbase, P, d.kc, d.xc, d.xv, d.xh);
+    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_QA, false, WRITE_QV>),
+               9, d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+               d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
+}
+
+static void search_pass(const DevBuffers &d, Launcher &L) {
+  PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene, d.xa,
+             d.cell_start, d.xb, d.rho, d.nl, d.ncnt, d.rbase, L.sh.slots);
 }
 
 static void step_paper(const DevBuffers &d, Launcher &L) {
@@ -1624,37 +1717,59 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
   sort_stage<kPaper>(d, L, 1, false);
   const float4 *XS = d.xa;
   const int P = sh.slots;
-  // iteration 0's lambda pass is the FP32 search; mixed precision runs the
-  // lambda passes of iterations 1 .. fp16_iterations over FP16 positions
-  auto fp16 = [&](int it) { return it >= 1 && it <= sh.fp16_iterations; };
+  // mixed precision: the lambda passes of iterations 1 .. m - 1 (m =
+  // fp16_iterations; iteration 0's is the FP32 search) gather 16-bit
+  // positions, and so do XSPH and the vorticity pass; every delta pass
+  // gathers FP32 records and updates FP32 positions
+  const int m = sh.fp16_iterations;
   for (int it = 0; it < sh.iterations; ++it) {
     if (it == 0) {
-      PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene,
-                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, d.rbase, P);
-    } else if (fp16(it)) {
-      PBF_GATHER(L, kClsLambda, k_lambda_list_h, 9, d.scenes, d.block_scene,
-                 XS, d.xc, d.xh, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
+      search_pass(d, L);
+    } else if (it < m) {
+      PBF_GATHER(L, kClsLambda, k_lambda_list_q, 9, d.scenes, d.block_scene,
+                 XS, d.xc, d.qa, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb,
+                 d.rho);
     } else {
       PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
                  XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
     }
-    if (it == sh.iterations - 1)
-      delta_pass<true, false>(d, L);
-    else if (fp16(it + 1))
-      delta_pass<false, true>(d, L);
-    else
-      delta_pass<false, false>(d, L);
+    if (it == sh.iterations - 1) {
+      if (m > 0)
+        delta_pass<true, false, true>(d, L);
+      else
+        delta_pass<true, false, false>(d, L);
+    } else if (it + 1 < m) {
+      delta_pass<false, true, false>(d, L);
+    } else {
+      delta_pass<false, false, false>(d, L);
+    }
   }
   if (sh.vorticity) {
-    PBF_GATHER(L, kClsXsph, k_xsph_omega, 9, d.scenes, d.block_scene, XS,
-               d.xv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx, d.om, d.xb);
-    PBF_GATHER(L, kClsVort, k_vort_final, 9, d.scenes, d.block_scene, XS,
-               d.xb, d.cell_start, d.vx, d.om, d.nl, d.ncnt, d.rbase, P,
-               d.pid_s, d.pos, d.vel, d.pid, d.err, next_sort(d));
+    if (m > 0) {
+      PBF_GATHER(L, kClsXsph, k_xsph_omega<true>, 9, d.scenes, d.block_scene,
+                 XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx,
+                 d.om, d.xb, d.qa);
+      PBF_GATHER(L, kClsVort, k_vort_final<true>, 9, d.scenes, d.block_scene,
+                 XS, d.xb, d.xv, d.qa, d.cell_start, d.vx, d.om, d.nl, d.ncnt,
+                 d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err,
+                 next_sort(d));
+    } else {
+      PBF_GATHER(L, kClsXsph, k_xsph_omega<false>, 9, d.scenes, d.block_scene,
+                 XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx,
+                 d.om, d.xb, d.qa);
+      PBF_GATHER(L, kClsVort, k_vort_final<false>, 9, d.scenes, d.block_scene,
+                 XS, d.xb, d.xv, d.qa, d.cell_start, d.vx, d.om, d.nl, d.ncnt,
+                 d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err,
+                 next_sort(d));
+    }
+  } else if (m > 0) {
+    PBF_GATHER(L, kClsXsph, k_xsph_final_paper<true>, 9, d.scenes,
+               d.block_scene, XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt,
+               d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err, next_sort(d));
   } else {
-    PBF_GATHER(L, kClsXsph, k_xsph_final_paper, 9, d.scenes, d.block_scene,
-               XS, d.xv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.pid_s, d.pos,
-               d.vel, d.pid, d.err, next_sort(d));
+    PBF_GATHER(L, kClsXsph, k_xsph_final_paper<false>, 9, d.scenes,
+               d.block_scene, XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt,
+               d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err, next_sort(d));
   }
 }
 
@@ -1695,13 +1810,12 @@ bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                    d.block_scene, d.xa, d.cell_start, d.lam, d.rho, d.xold,
                    d.xb, d.v);
     } else if (cls == kClsSearch) {
-      PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene,
-                 XS, d.cell_start, d.xb, d.rho, d.nl, d.ncnt, d.rbase, P);
+      search_pass(d, L);
     } else if (cls == kClsLambda) {
       PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
                  XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
     } else {
-      delta_pass<false, false>(d, L);
+      delta_pass<false, false, false>(d, L);
     }
   }
   return true;

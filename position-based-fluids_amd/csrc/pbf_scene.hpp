@@ -36,6 +36,14 @@ constexpr int kGroupSkip = 128 - kPerGroup * kEntryBits;
 constexpr int kMaxGroups = 9;
 constexpr int kMaxHits = kMaxGroups * kPerGroup;
 
+// Mixed precision: neighbour positions as 16-bit fixed point in units of
+// h / 2^kQBits from the scene's lower corner, kept modulo 2^16 (a period of
+// 2^(16 - kQBits) h = 16 h): the difference of two such coordinates, taken
+// modulo 2^16 as a signed 16-bit value, is the exact difference of the
+// quantised positions for any two points less than 8 h apart (a listed pair
+// is within h at x* and moves far less than 7 h within a step).
+constexpr int kQBits = 12;
+
 // Per-scene constants, resident in device memory and read by the kernels
 // with scalar loads (the scene index is uniform per work-group).  REF
 // scenes hold the 6-significant-digit constants of src/main.cpp:149-168.
@@ -58,7 +66,9 @@ struct Scene {
   float wave;          // x-min wall offset (Simulation::setWaveGenerator)
   float sphere[4];     // PAPER obstacle (cx, cy, cz, r), r <= 0: none
   float reach2;        // PAPER: ((1 + kReachMargin) h)^2
-  int pad[3];
+  float q_scale;       // mixed precision: 2^kQBits / h
+  float q_unit;        // mixed precision: h / 2^kQBits
+  int pad[1];
 };
 static_assert(sizeof(Scene) % 16 == 0, "Scene must stay 16-byte aligned");
 
@@ -97,8 +107,11 @@ struct DevBuffers {
   XVRec *xv;      // PAPER: (x, rho), (v, m) after the last delta pass
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
-  uint2 *xh;      // mixed precision: FP16 x - bin origin (sorted)
-  int *kc;        // mixed precision: cell id of each sorted slot
+  // mixed precision records (sorted), 16-bit fixed-point x (x | y << 16,
+  // z | payload << 16) and FP16 payloads:
+  uint2 *qa;      //   x after a delta pass (payload 0), for a lambda pass;
+                  //   x and |omega| after XSPH, for the vorticity pass
+  uint4 *qv;      //   x, rho / rho0, v after the last delta pass, for XSPH
   uint4 *nl;      // PAPER neighbour lists: kMaxGroups groups of kPerGroup
                   // entries per slot, group g of slot i at [g * P + i]
   int *ncnt;      // list length per slot (> kMaxHits: list overflowed)

@@ -257,7 +257,8 @@ def test_paper_list_overflow_matches_oracle(spacing, dims, precision):
 
 
 # ----------------------------------------------------------------------------
-# mixed precision (FP16 cell-relative gathers in the leading lambda passes)
+# mixed precision (16-bit neighbour records in the leading iterations and in
+# the XSPH and vorticity passes)
 # ----------------------------------------------------------------------------
 
 def mixed(p, fp32_iterations=1):
@@ -266,11 +267,12 @@ def mixed(p, fp32_iterations=1):
     return q
 
 
-# An FP16 offset inside a cell of length h = 0.1 is rounded by at most
-# 0.1 * 2^-12 = 2.4e-5 m; it enters only the lambda values of the FP16
-# passes (the position updates read FP32 positions), so after 10 steps the
-# mixed run stays within 2e-4 m (2e-3 h) of the FP64 oracle, twice the FP32
-# bar above.
+# A 16-bit neighbour coordinate is rounded by at most h / 2^13 = 1.2e-5 m
+# (units of h / 2^12) and FP16 lambdas, densities and velocities by 2^-11
+# relative; they enter the pair terms of the 16-bit passes, while every
+# position update and the last FP32 iteration work on FP32 positions, so
+# after 10 steps the mixed run stays within 2e-4 m (2e-3 h) of the FP64
+# oracle, twice the FP32 bar above.
 MIXED_POS_TOL = 2e-4
 
 
@@ -280,7 +282,7 @@ def test_mixed_vs_fp64_oracle():
     p, parts = sc.params[0], sc.particles[0]
     say("mixed vs fp64", sc.n)
     with pbf.Simulation(mixed(p), parts) as s:
-        assert s.config()[3] == p.iterations - 2   # iterations 1 .. I - 2
+        assert s.config()[3] == p.iterations - 1   # iterations 0 .. I - 2
         s.step(10)
         gm, gvm = s.dump()
         dm = s.diagnostics()
