@@ -372,6 +372,8 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   d.perm = S.alloc<int>(P);
   d.ppid = S.alloc<int>(P);
   d.pid_s = S.alloc<int>(P);
+  d.dense = S.alloc<int>(2 + 2 * (P / pbf::kDenseCell + 1));
+  d.dsc = S.alloc<int2>(P);
   d.xa = S.alloc<float4>(P);
   d.xb = S.alloc<float4>(P);
   if (sh.physics == pbf::kPaper) {
@@ -426,6 +428,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   PBF_HIP(hipMemset(d.count, 0, Mpad * sizeof(int)));
   PBF_HIP(hipMemset(d.partial, 0, Tpad * sizeof(int)));
   PBF_HIP(hipMemset(d.err, 0, sizeof(int)));
+  PBF_HIP(hipMemset(d.dense, 0, 2 * sizeof(int)));
   PBF_HIP(hipMemcpy(d.block_scene, S.block_scene.data(),
                     S.block_scene.size() * sizeof(int),
                     hipMemcpyHostToDevice));

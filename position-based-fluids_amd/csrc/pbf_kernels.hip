@@ -524,8 +524,12 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
 }
 
 __global__ __launch_bounds__(256) void k_scan_reduce(
-    const int4 *__restrict__ count, int *__restrict__ partial) {
+    const int4 *__restrict__ count, int *__restrict__ partial,
+    int *__restrict__ dense) {
   __shared__ int lds[4];
+  // the dense-cell list of this step's reorder starts empty (its last
+  // reader, the previous step's k_reorder_dense, has finished)
+  if (blockIdx.x == 0 && threadIdx.x < 2) dense[threadIdx.x] = 0;
   const int4 c = count[blockIdx.x * 256 + threadIdx.x];
   int tot;
   block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot);
@@ -643,13 +647,15 @@ __global__ __launch_bounds__(256) void k_scatter(
 
 // Arrival order within a cell is not deterministic; rank each slot among
 // its cell by particle id and move it there, gathering x*, x and the id.
+// A cell of more than kDenseCell slots is listed in dense (count, cursor,
+// then (first slot, end) pairs) and left to k_reorder_dense.
 __global__ __launch_bounds__(256) void k_reorder(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const int *__restrict__ perm, const int *__restrict__ ppid,
     const int *__restrict__ key, const int *__restrict__ cell_start,
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
     int *__restrict__ pid_s, float4 *__restrict__ xa,
-    float4 *__restrict__ xold) {
+    float4 *__restrict__ xold, int *__restrict__ dense) {
   const int lb = logical_block();
   const Scene &sc = scenes[scene_of(block_scene, lb)];
   const int t = lb * blockDim.x + threadIdx.x;
@@ -661,6 +667,14 @@ __global__ __launch_bounds__(256) void k_reorder(
   // k_scatter reports, reach past it)
   const int lo = max(cell_start[c] + sc.shift, sc.slot_off);
   const int hi = min(cell_start[c + 1] + sc.shift, sc.slot_off + sc.n);
+  if (hi - lo > kDenseCell) {
+    // ranked by k_reorder_dense; the cell's first slot lists it
+    if (t == lo) {
+      const int k = atomicAdd(&dense[0], 1);
+      reinterpret_cast<int2 *>(dense + 2)[k] = make_int2(lo, hi);
+    }
+    return;
+  }
   int r = 0;
   for (int u = lo; u < hi; ++u) r += (ppid[u] < p) ? 1 : 0;
   const int f = lo + r;
@@ -669,6 +683,82 @@ __global__ __launch_bounds__(256) void k_reorder(
   pid_s[f] = p;
   xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
   xold[f] = make_float4(x.x, x.y, x.z, pu.w);
+}
+
+// The rank of each slot of a dense cell among the cell by particle id, in
+// O(m) for ids spread over their range instead of the O(m^2) count of
+// k_reorder: the cell's ids are counted into kDenseBuckets buckets of equal
+// width over [min id, max id], the counts scanned to bucket starts, the
+// slots placed bucket by bucket into scratch (order within a bucket
+// arbitrary), and a slot's rank is its bucket's start plus the number of
+// smaller ids in its bucket.  Work-groups take listed cells from a cursor
+// until the list is exhausted.  Writes what k_reorder writes.
+__global__ __launch_bounds__(1024) void k_reorder_dense(
+    const int *__restrict__ perm, const int *__restrict__ ppid,
+    const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
+    int *__restrict__ pid_s, float4 *__restrict__ xa,
+    float4 *__restrict__ xold, int *__restrict__ dense,
+    int2 *__restrict__ scratch) {
+  __shared__ int cnt[kDenseBuckets], start[kDenseBuckets], cur[kDenseBuckets];
+  __shared__ int s_item, s_min, s_max;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int ncell = dense[0];
+  const int2 *cells = reinterpret_cast<const int2 *>(dense + 2);
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd(&dense[1], 1);
+    if (tid == 0) {
+      s_min = INT_MAX;
+      s_max = INT_MIN;
+    }
+    for (int b = tid; b < kDenseBuckets; b += nt) cnt[b] = cur[b] = 0;
+    __syncthreads();
+    const int item = s_item;
+    if (item >= ncell) return;  // uniform: every work-group reaches it
+    const int lo = cells[item].x, hi = cells[item].y;
+    int mn = INT_MAX, mx = INT_MIN;
+    for (int t = lo + tid; t < hi; t += nt) {
+      mn = min(mn, ppid[t]);
+      mx = max(mx, ppid[t]);
+    }
+    atomicMin(&s_min, mn);
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    const int pmin = s_min;
+    const int width = (s_max - pmin) / kDenseBuckets + 1;
+    for (int t = lo + tid; t < hi; t += nt)
+      atomicAdd(&cnt[(ppid[t] - pmin) / width], 1);
+    __syncthreads();
+    if (tid == 0) {
+      int run = 0;
+      for (int b = 0; b < kDenseBuckets; ++b) {
+        start[b] = run;
+        run += cnt[b];
+      }
+    }
+    __syncthreads();
+    for (int t = lo + tid; t < hi; t += nt) {
+      const int p = ppid[t];
+      const int b = (p - pmin) / width;
+      scratch[lo + start[b] + atomicAdd(&cur[b], 1)] = make_int2(p, t);
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int q = lo + tid; q < hi; q += nt) {
+      const int2 e = scratch[q];
+      const int b = (e.x - pmin) / width;
+      const int b0 = lo + start[b], b1 = b0 + cnt[b];
+      int r = start[b];
+      for (int u = b0; u < b1; ++u) r += scratch[u].x < e.x ? 1 : 0;
+      const int f = lo + r;
+      const int src = perm[e.y];
+      const float4 pu = pred_u[src];
+      const float4 x = pos[src];
+      pid_s[f] = e.x;
+      xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+      xold[f] = make_float4(x.x, x.y, x.z, pu.w);
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1648,7 +1738,7 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
                d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank, d.count,
                predict);
   PBF_LAUNCH(L, kClsScan, k_scan_reduce, sh.scan_blocks, 256,
-             reinterpret_cast<const int4 *>(d.count), d.partial);
+             reinterpret_cast<const int4 *>(d.count), d.partial, d.dense);
   if (sh.scan_blocks <= kScanSumTiles) {
     PBF_LAUNCH(L, kClsScan, k_scan_down_sum, sh.scan_blocks, 256,
                reinterpret_cast<const int4 *>(d.count), d.partial,
@@ -1665,7 +1755,9 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
              d.err);
   PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
              d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
-             d.xa, d.xold);
+             d.xa, d.xold, d.dense);
+  PBF_LAUNCH(L, kClsReorder, k_reorder_dense, kDenseBlocks, 1024, d.perm,
+             d.ppid, d.pred_u, d.pos, d.pid_s, d.xa, d.xold, d.dense, d.dsc);
 }
 
 static NextSort next_sort(const DevBuffers &d) {

@@ -26,6 +26,13 @@ constexpr int kTopThreads = 1024;
 constexpr int kTopIpt = 16;
 constexpr int kTopChunk = kTopThreads * kTopIpt;
 
+// Cells of more than kDenseCell slots are ranked by id in k_reorder_dense
+// (kDenseBlocks work-groups, kDenseBuckets id buckets per cell) instead of
+// k_reorder's count over the cell.
+constexpr int kDenseCell = 256;
+constexpr int kDenseBlocks = 64;
+constexpr int kDenseBuckets = 4096;
+
 // kernel classes reported by the profiler
 enum KernelClass {
   kClsPredict = 0,

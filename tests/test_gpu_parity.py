@@ -443,6 +443,57 @@ def test_all_particles_in_one_cell():
     assert np.array_equal(gp, op) and np.array_equal(gv, ov)
 
 
+def test_dense_cell_rank_matches_oracle():
+    """1,000 particles in one cell: more than k_reorder ranks by counting
+    (256), so the cell goes to the bucketed rank of k_reorder_dense; the
+    sort and three REF steps are bit-exact against the oracle."""
+    say("edge: dense cell, 1000 particles")
+    p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF)
+    rng = np.random.default_rng(17)
+    parts = np.zeros((1000, 10), np.float32)
+    parts[:, 0] = 1.0
+    parts[:, 1:4] = (0.5 + rng.uniform(0.0, 0.09, (1000, 3))).astype(np.float32)
+    with pbf.Simulation(p, parts) as s:
+        g = s.probe()
+    pos, vel = orc.state_from_particles(parts)
+    o = orc.probe_f32(p, pos, vel, 1000)
+    assert np.bincount(o["cell_of"]).max() > 256
+    for k in ("cell_of", "sorted_ids", "cell_start"):
+        assert np.array_equal(g[k], o[k]), k
+    gp, gv = run_gpu(p, parts, 3)
+    op, ov = orc.run_f32(p, pos, vel, 3)
+    assert np.array_equal(gp, op) and np.array_equal(gv, ov)
+
+
+def test_hundred_thousand_particles_in_eight_cells():
+    """100,000 particles in a box of 2 x 2 x 2 cells (12,500 per cell): the
+    sort's permutation is numpy's stable sort of the ids by cell and the cell
+    table the scan of the counts.  k_reorder's count over a cell would be
+    O(m^2) (1.6e8 id comparisons per cell); the bucketed rank of
+    k_reorder_dense is O(m) for spread ids: the reorder class of one step
+    takes about 0.1 ms here (kernel events), asserted below 5 ms."""
+    say("edge: 100000 particles in 8 cells")
+    p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF)
+    p.x_n = p.y_n = p.z_n = 2.0
+    rng = np.random.default_rng(23)
+    n = 100_000
+    parts = np.zeros((n, 10), np.float32)
+    parts[:, 0] = 1.0
+    parts[:, 1:4] = rng.uniform(0.01, 0.99, (n, 3)).astype(np.float32)
+    with pbf.Simulation(p, parts) as s:
+        g = s.probe()
+        prof = s.profile(1)
+    cell = g["cell_of"]
+    assert cell.max() < 8 and np.bincount(cell).min() > 10_000
+    order = np.argsort(cell, kind="stable")
+    assert np.array_equal(g["sorted_ids"], order)
+    assert np.array_equal(g["cell_start"],
+                          np.searchsorted(cell[order], np.arange(9), side="left"))
+    ms, launches = prof["reorder"]
+    say("reorder ms (one step):", ms, "launches", launches)
+    assert ms < 5.0
+
+
 @pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
 def test_nan_particle_is_reported(physics):
     """A non-finite input position bins into cell (0, 0, 0) (the cell
