@@ -173,6 +173,25 @@ int pbf_read_par(const char *path, pbf_params *out, char *part_file,
 int pbf_read_particles(const char *path, pbf_particle *out, uint32_t cap,
                        uint32_t *n);
 
+/* Advance caller-owned device state nsteps steps, allocating nothing: no
+ * reference counterpart; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  pos4 / vel4 are float4[n] arrays in DEVICE memory of the
+ * current device, in original particle order: pos = (x, y, z, any), vel =
+ * (vx, vy, vz, m) in; the state after nsteps steps out, laid out as
+ * pbf_dump returns it (pos.w = |v|).  One scene; the same kernels and
+ * results as pbf_create + pbf_step + pbf_dump with the same parameters
+ * (work-group size block_size, or 128 if 0; no autotuning).  Every
+ * temporary lives in the library's static device scratch (16 MiB per GPU,
+ * held from library load to process end): the 32,768-particle dam break
+ * fits; sizes that need more are refused with PBF_ERR_INVALID.  The work
+ * is queued on `stream` (a hipStream_t; NULL: the null stream) and not
+ * waited for; it reports no numeric error (use a handle for that).  All
+ * calls on one GPU share its scratch: calls that overlap on one GPU (other
+ * streams, other threads) race. */
+int pbf_step_noalloc(const pbf_params *params, float *pos4, float *vel4,
+                     uint32_t n, uint32_t nsteps, void *stream);
+
 /* ---- parity hooks and measurement (no reference counterpart) ----------- */
 
 /* Run the neighbour-search stage of a step on the current state without

@@ -19,6 +19,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pbf.h"
@@ -290,6 +291,91 @@ struct pbf_sim {
 
 namespace {
 
+// Shape of a step over scenes of `scenes` (laid out, slot_off and cell_base
+// set) with the uniform settings of p0; work-group sizes untuned.
+pbf::StepShape make_shape(const pbf_params &p0, const pbf::Scene *scenes,
+                          size_t nscenes, const pbf_params *params,
+                          long long slots, long long cells) {
+  pbf::StepShape sh{};
+  sh.slots = (int)slots;
+  sh.cells = (int)cells;
+  sh.scan_blocks = (int)((cells + pbf::kScanTile) / pbf::kScanTile);
+  sh.physics = p0.physics == PBF_PHYSICS_REF ? pbf::kRef : pbf::kPaper;
+  sh.iterations = p0.iterations;
+  // the lambda passes of iterations 1 .. I - F - 1, XSPH and the vorticity
+  // pass gather 16-bit records; iteration 0's search, every delta pass and
+  // the last F iterations are FP32
+  sh.fp16_iterations = p0.precision == PBF_PRECISION_MIXED
+                           ? std::max(p0.iterations - p0.fp32_iterations, 0)
+                           : 0;
+  bool vort = false;
+  for (size_t s = 0; s < nscenes; ++s)
+    vort = vort || params[s].vorticity_eps != 0.0f;
+  sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
+  sh.scorr_n4 = 1;
+  for (size_t s = 0; s < nscenes; ++s) sh.scorr_n4 &= scenes[s].scorr_n == 4;
+  for (int c = 0; c < pbf::kNumClasses; ++c)
+    sh.block[c] = p0.block_size ? p0.block_size : 128;
+  return sh;
+}
+
+// Every device buffer of a step of shape sh over nscenes scenes, placed by
+// get(bytes) (an allocation, or a bump pointer into preallocated memory).
+template <typename Get>
+void layout_buffers(pbf::DevBuffers &d, const pbf::StepShape &sh,
+                    size_t nscenes, Get &&get) {
+  auto take = [&](auto *&ptr, size_t count) {
+    using T = std::remove_reference_t<decltype(*ptr)>;
+    ptr = static_cast<T *>(get(std::max<size_t>(count, 1) * sizeof(T)));
+  };
+  const size_t P = (size_t)sh.slots;
+  take(d.pos, P);
+  take(d.vel, P);
+  take(d.pid, P);
+  take(d.pred_u, P);
+  take(d.key, P);
+  take(d.rank, P);
+  const size_t Mpad = (size_t)sh.scan_blocks * pbf::kScanTile;  // >= M + 1
+  take(d.count, Mpad);
+  take(d.cell_start, Mpad);
+  const size_t Tpad = ((size_t)sh.scan_blocks + pbf::kTopChunk) /
+                      pbf::kTopChunk * pbf::kTopChunk;  // whole scan rounds
+  take(d.partial, Tpad);
+  take(d.perm, P);
+  take(d.ppid, P);
+  take(d.pid_s, P);
+  take(d.dense, 2 + 2 * (P / pbf::kDenseCell + 1));
+  take(d.dsc, P);
+  take(d.xa, P);
+  take(d.xb, P);
+  if (sh.physics == pbf::kPaper) {
+    take(d.xc, P);
+    take(d.xv, P);
+  }
+  take(d.xold, P);
+  take(d.lam, P);
+  take(d.rho, P);
+  if (sh.physics == pbf::kRef) take(d.v, P);
+  if (sh.vorticity) {
+    take(d.vx, P);
+    take(d.om, P);
+  }
+  if (sh.fp16_iterations > 0) {
+    take(d.qa, P);
+    take(d.qv, P);
+  }
+  if (sh.physics == pbf::kPaper) {
+    take(d.nl, P * (size_t)pbf::kMaxGroups);
+    take(d.ncnt, P);
+    take(d.rbase, P * 9);
+  }
+  take(d.nbr, P);
+  take(d.red, (P / 64 + 1) * 8);
+  take(d.err, 1);
+  take(d.block_scene, P / pbf::kSlotAlign);
+  take(d.scenes, nscenes);
+}
+
 void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
            const pbf_particle *const *parts, const uint32_t *counts) {
   if (nscenes == 0) invalid("nscenes must be >= 1");
@@ -334,74 +420,17 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     invalid("problem too large for PAPER physics (more than 2^26 slots)");
   const pbf_params &p0 = params[0];
   pbf::StepShape &sh = S.shape;
-  sh.slots = (int)slot;
-  sh.cells = (int)cells;
-  sh.scan_blocks = (int)((cells + pbf::kScanTile) / pbf::kScanTile);
-  sh.physics = p0.physics == PBF_PHYSICS_REF ? pbf::kRef : pbf::kPaper;
-  sh.iterations = p0.iterations;
-  // the lambda passes of iterations 1 .. I - F - 1, XSPH and the vorticity
-  // pass gather 16-bit records; iteration 0's search, every delta pass and
-  // the last F iterations are FP32
-  sh.fp16_iterations = p0.precision == PBF_PRECISION_MIXED
-                           ? std::max(p0.iterations - p0.fp32_iterations, 0)
-                           : 0;
-  bool vort = false;
-  for (uint32_t s = 0; s < nscenes; ++s)
-    vort = vort || params[s].vorticity_eps != 0.0f;
-  sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
-  sh.scorr_n4 = 1;
-  for (const pbf::Scene &sc : S.scenes) sh.scorr_n4 &= sc.scorr_n == 4;
-  for (int c = 0; c < pbf::kNumClasses; ++c)
-    sh.block[c] = p0.block_size ? p0.block_size : 128;
+  sh = make_shape(p0, S.scenes.data(), S.scenes.size(), params, slot, cells);
   S.tune_pending = p0.block_size == 0;
 
   const size_t P = (size_t)sh.slots;
   pbf::DevBuffers &d = S.d;
-  d.pos = S.alloc<float4>(P);
-  d.vel = S.alloc<float4>(P);
-  d.pid = S.alloc<int>(P);
-  d.pred_u = S.alloc<float4>(P);
-  d.key = S.alloc<int>(P);
-  d.rank = S.alloc<int>(P);
-  const size_t Mpad = (size_t)sh.scan_blocks * pbf::kScanTile;  // >= M + 1
-  d.count = S.alloc<int>(Mpad);
-  d.cell_start = S.alloc<int>(Mpad);
+  layout_buffers(d, sh, nscenes, [&](size_t bytes) {
+    return static_cast<void *>(S.alloc<char>(bytes));
+  });
+  const size_t Mpad = (size_t)sh.scan_blocks * pbf::kScanTile;
   const size_t Tpad = ((size_t)sh.scan_blocks + pbf::kTopChunk) /
-                      pbf::kTopChunk * pbf::kTopChunk;  // whole scan rounds
-  d.partial = S.alloc<int>(Tpad);
-  d.perm = S.alloc<int>(P);
-  d.ppid = S.alloc<int>(P);
-  d.pid_s = S.alloc<int>(P);
-  d.dense = S.alloc<int>(2 + 2 * (P / pbf::kDenseCell + 1));
-  d.dsc = S.alloc<int2>(P);
-  d.xa = S.alloc<float4>(P);
-  d.xb = S.alloc<float4>(P);
-  if (sh.physics == pbf::kPaper) {
-    d.xc = S.alloc<float4>(P);
-    d.xv = S.alloc<pbf::XVRec>(P);
-  }
-  d.xold = S.alloc<float4>(P);
-  d.lam = S.alloc<float>(P);
-  d.rho = S.alloc<float>(P);
-  if (sh.physics == pbf::kRef) d.v = S.alloc<float4>(P);
-  if (sh.vorticity) {
-    d.vx = S.alloc<float4>(P);
-    d.om = S.alloc<float4>(P);
-  }
-  if (sh.fp16_iterations > 0) {
-    d.qa = S.alloc<uint2>(P);
-    d.qv = S.alloc<uint4>(P);
-  }
-  if (sh.physics == pbf::kPaper) {
-    d.nl = S.alloc<uint4>(P * (size_t)pbf::kMaxGroups);
-    d.ncnt = S.alloc<int>(P);
-    d.rbase = S.alloc<int>(P * 9);
-  }
-  d.nbr = S.alloc<int>(P);
-  d.red = S.alloc<double>((P / 64 + 1) * 8);
-  d.err = S.alloc<int>(1);
-  d.block_scene = S.alloc<int>(S.block_scene.size());
-  d.scenes = S.alloc<pbf::Scene>(nscenes);
+                      pbf::kTopChunk * pbf::kTopChunk;
   S.diag_out = S.alloc<double>(8);
   PBF_HIP(hipHostMalloc((void **)&S.scenes_pinned,
                         nscenes * sizeof(pbf::Scene), hipHostMallocDefault));
@@ -961,6 +990,59 @@ int pbf_time_kernel(pbf_sim *S, const char *cls, uint32_t reps, double *ms) {
       throw;
     }
     cleanup();
+  });
+}
+
+int pbf_step_noalloc(const pbf_params *params, float *pos4, float *vel4,
+                     uint32_t n, uint32_t nsteps, void *stream) {
+  return guard([&] {
+    need(params != nullptr, "params is NULL");
+    validate(*params);
+    if (n == 0 || nsteps == 0) return;
+    need(pos4 && vel4, "pos4/vel4 is NULL");
+    pbf::Scene sc = make_scene(*params);
+    sc.n = (int)n;  // one scene from slot 0: offsets, shift, bases 0
+    const long long slots =
+        ((long long)n + pbf::kSlotAlign - 1) / pbf::kSlotAlign *
+        pbf::kSlotAlign;
+    if (slots > (1LL << 30)) invalid("problem too large (slots beyond 2^30)");
+    pbf::StepShape sh = make_shape(*params, &sc, 1, params, slots, sc.ncells);
+    // the buffers of the step, 256-byte aligned, from the start of the
+    // scratch: first their extent, then their places
+    size_t need_bytes = 0;
+    auto bump = [&](size_t base) {
+      return [&need_bytes, base](size_t bytes) {
+        const size_t at = (need_bytes + 255) / 256 * 256;
+        need_bytes = at + bytes;
+        return reinterpret_cast<void *>(base + at);
+      };
+    };
+    pbf::DevBuffers d{};
+    layout_buffers(d, sh, 1, bump(0));
+    if (need_bytes > pbf::kStaticScratchBytes) {
+      char msg[256];
+      std::snprintf(msg, sizeof msg,
+                    "pbf_step_noalloc: %u particles need %zu bytes of device "
+                    "scratch; the static scratch holds %zu bytes; use "
+                    "pbf_create",
+                    n, need_bytes, pbf::kStaticScratchBytes);
+      invalid(msg);
+    }
+    void *base = nullptr;
+    PBF_HIP(pbf::static_scratch(&base));
+    need_bytes = 0;
+    layout_buffers(d, sh, 1, bump(reinterpret_cast<size_t>(base)));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    pbf::enqueue_static_setup(d, sh, sc, st);
+    pbf::enqueue_load_state(d, sh, reinterpret_cast<const float4 *>(pos4),
+                            reinterpret_cast<const float4 *>(vel4), (int)n,
+                            st);
+    pbf::enqueue_predict_zeroed(d, sh, st);
+    for (uint32_t t = 0; t < nsteps; ++t)
+      pbf::enqueue_step(d, sh, st, nullptr);
+    pbf::enqueue_store_state(d, sh, reinterpret_cast<float4 *>(pos4),
+                             reinterpret_cast<float4 *>(vel4), (int)n, st);
+    PBF_HIP(hipGetLastError());
   });
 }
 

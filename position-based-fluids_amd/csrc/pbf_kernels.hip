@@ -1923,6 +1923,11 @@ static void clear_counts(const DevBuffers &d, const StepShape &sh,
 void enqueue_predict(const DevBuffers &d, const StepShape &sh,
                      hipStream_t st) {
   clear_counts(d, sh, st);
+  enqueue_predict_zeroed(d, sh, st);
+}
+
+void enqueue_predict_zeroed(const DevBuffers &d, const StepShape &sh,
+                            hipStream_t st) {
   const int bs = 256, grid = sh.slots / bs;
   if (sh.physics == kRef)
     k_predict_count<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.pos,
@@ -1932,6 +1937,76 @@ void enqueue_predict(const DevBuffers &d, const StepShape &sh,
     k_predict_count<kPaper><<<grid, bs, 0, st>>>(
         d.scenes, d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank,
         d.count, 1);
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int *__restrict__ a,
+                                                  int value, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    a[i] = value;
+}
+
+__global__ void k_set_scene(Scene *__restrict__ out, Scene sc) { *out = sc; }
+
+void enqueue_static_setup(const DevBuffers &d, const StepShape &sh,
+                          const Scene &scene, hipStream_t st) {
+  k_set_scene<<<1, 1, 0, st>>>(d.scenes, scene);
+  auto fill = [&](int *a, int v, size_t n) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 1024);
+    k_fill_int<<<std::max(grid, 1), 256, 0, st>>>(a, v, n);
+  };
+  fill(d.block_scene, 0, (size_t)sh.slots / kSlotAlign);
+  fill(d.count, 0, (size_t)sh.scan_blocks * kScanTile);
+  fill(d.partial, 0,
+       ((size_t)sh.scan_blocks + kTopChunk) / kTopChunk * kTopChunk);
+  fill(d.err, 0, 1);
+  fill(d.dense, 0, 2);
+}
+
+// Slot k < n takes particle k: x with .w = 0, v with .w = m, id k; the
+// padding slots are zero with id -1.
+__global__ __launch_bounds__(256) void k_load_state(
+    const float4 *__restrict__ pos4, const float4 *__restrict__ vel4, int n,
+    int slots, float4 *__restrict__ pos, float4 *__restrict__ vel,
+    int *__restrict__ pid) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= slots) return;
+  if (k < n) {
+    const float4 x = pos4[k];
+    pos[k] = make_float4(x.x, x.y, x.z, 0.0f);
+    vel[k] = vel4[k];
+    pid[k] = k;
+  } else {
+    pos[k] = vel[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pid[k] = -1;
+  }
+}
+
+// The state of each slot back to its particle's place in original order.
+__global__ __launch_bounds__(256) void k_store_state(
+    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+    const int *__restrict__ pid, int n, float4 *__restrict__ pos4,
+    float4 *__restrict__ vel4) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;  // the particles occupy the first n slots
+  const int id = pid[k];
+  if ((unsigned)id >= (unsigned)n) return;
+  pos4[id] = pos[k];
+  vel4[id] = vel[k];
+}
+
+void enqueue_load_state(const DevBuffers &d, const StepShape &sh,
+                        const float4 *pos4, const float4 *vel4, int n,
+                        hipStream_t st) {
+  k_load_state<<<sh.slots / 256, 256, 0, st>>>(pos4, vel4, n, sh.slots,
+                                               d.pos, d.vel, d.pid);
+}
+
+void enqueue_store_state(const DevBuffers &d, const StepShape &sh,
+                         float4 *pos4, float4 *vel4, int n, hipStream_t st) {
+  k_store_state<<<(n + 255) / 256, 256, 0, st>>>(d.pos, d.vel, d.pid, n,
+                                                 pos4, vel4);
 }
 
 void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {

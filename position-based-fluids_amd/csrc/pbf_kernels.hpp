@@ -33,6 +33,11 @@ constexpr int kDenseCell = 256;
 constexpr int kDenseBlocks = 64;
 constexpr int kDenseBuckets = 4096;
 
+// pbf_step_noalloc: bytes of the static device scratch (per GPU) that
+// holds every temporary of the call (at 428 bytes per particle slot plus 8
+// per cell: the 32,768-particle dam break in PAPER physics uses 14 MB).
+constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
+
 // kernel classes reported by the profiler
 enum KernelClass {
   kClsPredict = 0,
@@ -87,6 +92,21 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st);
 void enqueue_diag(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                   double *out_dev);
 const char *kernel_class_name(int cls);
+// Device address of the static scratch on the current device.
+hipError_t static_scratch(void **ptr);
+// pbf_step_noalloc's set-up without memset or copies: the scene table and
+// block map filled by kernels, the counters the step needs at zero zeroed.
+void enqueue_static_setup(const DevBuffers &d, const StepShape &sh,
+                          const Scene &scene, hipStream_t st);
+// Caller state (original order, n particles) into the slots, and back.
+void enqueue_load_state(const DevBuffers &d, const StepShape &sh,
+                        const float4 *pos4, const float4 *vel4, int n,
+                        hipStream_t st);
+void enqueue_store_state(const DevBuffers &d, const StepShape &sh,
+                         float4 *pos4, float4 *vel4, int n, hipStream_t st);
+// enqueue_predict for counts already zeroed (no memset).
+void enqueue_predict_zeroed(const DevBuffers &d, const StepShape &sh,
+                            hipStream_t st);
 void enqueue_copy(const float4 *a, float4 *b, size_t n, hipStream_t st);
 
 }  // namespace pbf

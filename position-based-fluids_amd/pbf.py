@@ -132,6 +132,9 @@ _SIGNATURES = [
     ("pbf_get_config", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32)]),
     ("pbf_calibrate_copy", ctypes.c_int,
      [ctypes.c_uint64, ctypes.c_uint32, _P(ctypes.c_double)]),
+    ("pbf_step_noalloc", ctypes.c_int,
+     [_P(Params), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+      ctypes.c_uint32, ctypes.c_void_p]),
     ("pbf_read_par", ctypes.c_int,
      [ctypes.c_char_p, _P(Params), ctypes.c_char_p, ctypes.c_size_t]),
     ("pbf_read_particles", ctypes.c_int,

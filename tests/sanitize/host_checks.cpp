@@ -149,6 +149,31 @@ int main(int argc, char **argv) {
   check(pbf_create(nullptr, parts.data(), 1, &sim) == PBF_ERR_INVALID,
         "NULL params rejected");
 
+  // the entry point that allocates nothing: argument and size checks come
+  // before any device call
+  float dummy[8] = {0};
+  check(pbf_step_noalloc(nullptr, dummy, dummy, 1, 1, nullptr) ==
+            PBF_ERR_INVALID,
+        "noalloc: NULL params rejected");
+  check(pbf_step_noalloc(&bad, dummy, dummy, 1, 1, nullptr) ==
+            PBF_ERR_INVALID,
+        "noalloc: bad params rejected");
+  pbf_params big;
+  pbf_params_init(&big, PBF_PHYSICS_PAPER);
+  big.x_max = big.y_max = big.z_max = 4.0f;
+  big.x_n = big.y_n = big.z_n = 40.0f;
+  big.time_step_length = 0.01f;
+  big.rest_density = 6378.0f;
+  check(pbf_step_noalloc(&big, dummy, dummy, 0, 1, nullptr) == PBF_OK,
+        "noalloc: no particles is a no-op");
+  check(pbf_step_noalloc(&big, nullptr, dummy, 5, 1, nullptr) ==
+            PBF_ERR_INVALID,
+        "noalloc: NULL state rejected");
+  check(pbf_step_noalloc(&big, dummy, dummy, 1000000, 1, nullptr) ==
+                PBF_ERR_INVALID &&
+            std::strstr(pbf_last_error(), "scratch") != nullptr,
+        "noalloc: beyond the static scratch rejected");
+
   std::printf("sanitized host checks: %d failure(s)\n", failures);
   return failures ? 1 : 0;
 }

@@ -18,6 +18,7 @@ ROOT = os.path.dirname(HERE)
 PKG = os.path.join(ROOT, "position-based-fluids_amd")
 CSRC = os.path.join(PKG, "csrc")
 KERNELS_OBJ = os.path.join(PKG, "_build", "pbf_kernels.o")
+STATIC_OBJ = os.path.join(PKG, "_build", "pbf_static.o")
 ROCM = "/opt/rocm"
 
 
@@ -45,7 +46,7 @@ def test_host_code_under_asan_and_ubsan(tmp_path, golden_dir):
                        check=True, timeout=600)
         objs.append(obj)
     subprocess.run(["g++", "-fsanitize=address,undefined", "-fopenmp", "-o", exe,
-                    *objs, KERNELS_OBJ, "-L" + os.path.join(ROCM, "lib"),
+                    *objs, KERNELS_OBJ, STATIC_OBJ, "-L" + os.path.join(ROCM, "lib"),
                     "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")],
                    check=True, timeout=600)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0",
