@@ -432,6 +432,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 4])
     ap.add_argument("--physics", choices=["paper", "ref"], default="paper")
+    ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32",
+                    help="precision of the main workload (the bench line's "
+                         "value is defined on fp32)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=60,
@@ -480,6 +483,8 @@ def main():
     params, parts = sc.params[0], sc.particles[0]
     if args.block:
         params.block_size = args.block
+    if args.precision == "mixed":
+        params.precision, params.fp32_iterations = pbf.PRECISION_MIXED, 1
     n = sc.n
     iters = params.iterations
     print(f"[bench] {sc.name}: {n} particles, {iters} iterations, "
@@ -544,11 +549,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.precision == "fp32" else "f32+f16",
         "data": "synthetic",
         "config": {
             "workload": (f"{sc.name}: dam break, {n} particles, "
-                         f"{iters} iterations, {args.physics} physics, FP32"),
+                         f"{iters} iterations, {args.physics} physics, "
+                         f"{'FP32' if args.precision == 'fp32' else 'mixed precision'}"),
             "particles": n,
             "cells": cells,
             "iterations": iters,

@@ -703,17 +703,19 @@ __global__ __launch_bounds__(1024) void k_reorder_dense(
   __shared__ int s_item, s_min, s_max;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int ncell = dense[0];
+  if (ncell == 0) return;  // the usual case: no dense cell this step
   const int2 *cells = reinterpret_cast<const int2 *>(dense + 2);
   for (;;) {
     if (tid == 0) s_item = atomicAdd(&dense[1], 1);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= ncell) return;  // uniform: every work-group reaches it
+    for (int b = tid; b < kDenseBuckets; b += nt) cnt[b] = cur[b] = 0;
     if (tid == 0) {
       s_min = INT_MAX;
       s_max = INT_MIN;
     }
-    for (int b = tid; b < kDenseBuckets; b += nt) cnt[b] = cur[b] = 0;
     __syncthreads();
-    const int item = s_item;
-    if (item >= ncell) return;  // uniform: every work-group reaches it
     const int lo = cells[item].x, hi = cells[item].y;
     int mn = INT_MAX, mx = INT_MIN;
     for (int t = lo + tid; t < hi; t += nt) {

@@ -30,7 +30,7 @@ constexpr int kTopChunk = kTopThreads * kTopIpt;
 // (kDenseBlocks work-groups, kDenseBuckets id buckets per cell) instead of
 // k_reorder's count over the cell.
 constexpr int kDenseCell = 256;
-constexpr int kDenseBlocks = 64;
+constexpr int kDenseBlocks = 16;
 constexpr int kDenseBuckets = 4096;
 
 // pbf_step_noalloc: bytes of the static device scratch (per GPU) that
