@@ -357,6 +357,54 @@ def _vs_oracle(x, ref) -> dict:
             "worst_fp64_x": [float(v) for v in ref[k, :3]]}
 
 
+def vs_oracle(params, parts, steps, threads: int) -> dict:
+    """The workload's own input stepped by the FP64 oracle on the host and by
+    a fresh GPU handle; at each step count of `steps` (ascending): position
+    error of the GPU state against the oracle's, and the FP64 density error
+    of both states; the FP32 oracle's own position error against FP64 beside
+    it.  REF physics: also whether the GPU state equals the FP32 oracle
+    restatement of the reference kernels bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pbf_oracle as orc
+
+    orc.set_threads(threads)
+    ref = params.physics == pbf.PHYSICS_REF
+    p64, v64 = orc.state_from_particles(parts, np.float64)
+    p32, v32 = orc.state_from_particles(parts)
+    out = {"particles": int(parts.shape[0]), "oracle_threads": threads,
+           "steps": {}}
+    t_oracle, done = 0.0, 0
+    with pbf.Simulation(params, parts) as sim:
+        for k in steps:
+            if k > done:
+                sim.step(k - done)
+            x, v = sim.dump()
+            t0 = time.perf_counter()
+            p64, v64 = orc.run_f64(params, p64, v64, k - done)
+            t_oracle += time.perf_counter() - t0
+            at = _vs_oracle(x, p64)
+            at["gpu"] = _density_errors(
+                orc.density_grid_f64(params, x.astype(np.float64)),
+                params.rest_density)
+            at["fp64"] = _density_errors(orc.density_grid_f64(params, p64),
+                                         params.rest_density)
+            # the FP32 restatement on the host, against FP64 too: how far
+            # FP32 rounding alone carries a trajectory by step k
+            p32, v32 = orc.run_f32(params, p32, v32, k - done)
+            e32 = _vs_oracle(p32, p64)
+            at["fp32_oracle_vs_fp64"] = {"rms": e32["rms"], "max": e32["max"],
+                                         "p99": e32["p99"]}
+            if ref:
+                at["fp32_oracle_bit_identical"] = bool(
+                    np.array_equal(x, p32) and np.array_equal(v, v32))
+            done = k
+            out["steps"][str(k)] = at
+            print(f"[bench] vs FP64 oracle at step {k}: rms {at['rms']:.2e} m, "
+                  f"max {at['max']:.2e} m", file=sys.stderr, flush=True)
+    out["oracle_seconds"] = t_oracle
+    return out
+
+
 def mixed_vs_plain(timed: int, oracle_threads: int, with_oracle: bool):
     """The 4,194,304-particle sphere dam (c4), PAPER physics: FP32 against
     mixed precision, both run 50 steps from the same input; the speed of
@@ -447,6 +495,9 @@ def main():
                     help="steps of the fresh-handle run timed from step 0 "
                          "(0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--vs-fp64", default="", metavar="K1,K2,...",
+                    help="compare the workload with the FP64 oracle after "
+                         "these step counts (host cores; off by default)")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="steps of the 256-scene batch vs sequential "
                          "comparison (0: skip)")
@@ -617,6 +668,11 @@ def main():
               file=sys.stderr, flush=True)
         out["batch"] = batch_vs_sequential(physics, args.batch_steps)
     threads = max(1, min(16, cpu_info()["usable"]))
+    if args.vs_fp64:
+        ks = sorted({int(k) for k in args.vs_fp64.split(",") if k.strip()})
+        print(f"[bench] {sc.name} against the FP64 oracle at steps {ks}",
+              file=sys.stderr, flush=True)
+        out["vs_fp64_oracle"] = vs_oracle(params, parts, ks, threads)
     if args.mixed_steps > 0 and physics == pbf.PHYSICS_PAPER:
         print(f"[bench] c4 mixed precision vs FP32, {args.mixed_steps} timed steps",
               file=sys.stderr, flush=True)
