@@ -134,6 +134,25 @@ int pbf_dump(pbf_sim *sim, float *pos4, float *vel4);
 /* Batched handle: dump one scene. */
 int pbf_batch_dump(pbf_sim *sim, uint32_t scene, float *pos4, float *vel4);
 
+/* Replaces the positions buffer the reference shares with GL: the
+ * CL buffer created on the viewer's VBO (src/Simulation.cpp:149-158),
+ * acquired and released around every step (:464-467, :570-574), into which
+ * updatePositions writes each particle's (x, |v|) at its index
+ * (src/kernels/update_positions.cl:9-12) and which the viewer draws
+ * (src/visual/visual.cpp:256-268, 319-323).  Writes the current (x, |v|) of
+ * every particle of `scene` (0 for a single-scene handle) into dst, float4 by
+ * particle index in input order -- the layout pbf_dump's pos4 has -- and
+ * returns when the write is done.  dst is device memory of the handle's GPU
+ * holding at least that scene's particle count of float4: a hipMalloc
+ * buffer, or the pointer hipGraphicsResourceGetMappedPointer gives for a GL
+ * buffer registered with hipGraphicsGLRegisterBuffer and mapped with
+ * hipGraphicsMapResources (unmap it before drawing).  Waits first for all
+ * work queued on the GPU, on any stream (the map or a fill of dst included),
+ * as the reference finishes GL before acquiring the buffer (:464).  A
+ * pointer HIP does not know as device memory of that GPU is rejected
+ * (PBF_ERR_INVALID). */
+int pbf_positions_to_device(pbf_sim *sim, uint32_t scene, void *dst);
+
 /* Replaces Simulation::setWaveGenerator (src/Simulation.hpp:105-108):
  * moving x-min wall offset used by the collision clamp
  * (src/kernels/compute_delta.cl:130-131).  Applies to every scene. */

@@ -18,7 +18,8 @@ Simulation::Simulation(const ConfigParameters &parameters,
       mParams(parameters.toPbfParams()),
       mParticles(particles),
       mSim(nullptr),
-      mWaveGenerator(0.0f) {
+      mWaveGenerator(0.0f),
+      mSharingBuffer(nullptr) {
   mSystemSizeMin.s[0] = parameters.xMin;
   mSystemSizeMin.s[1] = parameters.yMin;
   mSystemSizeMin.s[2] = parameters.zMin;
@@ -41,6 +42,8 @@ void Simulation::init(void) {
         "Simulation::init");
   if (mWaveGenerator != 0.0f)
     check(pbf_set_wave_generator(mSim, mWaveGenerator), "setWaveGenerator");
+  if (mSharingBuffer)
+    check(pbf_positions_to_device(mSim, 0, mSharingBuffer), "Simulation::init");
 }
 
 void Simulation::initCells(void) {}
@@ -49,6 +52,8 @@ void Simulation::step(void) {
   if (!mSim) throw std::runtime_error("Simulation::step before init()");
   check(pbf_step(mSim, 1), "Simulation::step");
   check(pbf_synchronize(mSim), "Simulation::step");
+  if (mSharingBuffer)
+    check(pbf_positions_to_device(mSim, 0, mSharingBuffer), "Simulation::step");
 }
 
 void Simulation::dumpData(Float4 *(&positions), Float4 *(&velocities)) {
@@ -66,4 +71,11 @@ void Simulation::setWaveGenerator(const double value) {
   mWaveGenerator = (float)value;
   if (mSim)
     check(pbf_set_wave_generator(mSim, mWaveGenerator), "setWaveGenerator");
+}
+
+void Simulation::setSharingBuffer(void *devicePositions) {
+  mSharingBuffer = devicePositions;
+  if (mSim && mSharingBuffer)
+    check(pbf_positions_to_device(mSim, 0, mSharingBuffer),
+          "setSharingBuffer");
 }

@@ -40,6 +40,15 @@ class Simulation {
 
   void setWaveGenerator(const double value);
 
+  // The positions buffer the reference shares with its GL viewer (the
+  // GLuint of its constructor, src/Simulation.cpp:149-158): device memory of
+  // getNumberParticles() float4, e.g. a GL buffer registered with
+  // hipGraphicsGLRegisterBuffer and mapped by the caller.  Once set, every
+  // step() leaves each particle's (x, |v|) at its index there, as
+  // updatePositions does (update_positions.cl:9-12); nullptr stops it.  The
+  // current state is written at once.
+  void setSharingBuffer(void *devicePositions);
+
   // The underlying C-ABI handle (nullptr before init()).
   pbf_sim *handle() const { return mSim; }
 
@@ -53,4 +62,5 @@ class Simulation {
   std::vector<Float4> mVelocities;
   pbf_sim *mSim;
   float mWaveGenerator;
+  void *mSharingBuffer;
 };

@@ -762,6 +762,37 @@ int pbf_batch_dump(pbf_sim *S, uint32_t scene, float *pos4, float *vel4) {
   });
 }
 
+int pbf_positions_to_device(pbf_sim *S, uint32_t scene, void *dst) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    need(dst != nullptr, "dst is NULL");
+    need(scene < S->scenes.size(), "scene index out of range");
+    PBF_HIP(hipSetDevice(S->device));
+    // the kernel writes dst: it must be device memory of this handle's GPU
+    // (a host pointer would fault the GPU instead of failing here)
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, dst) != hipSuccess) {
+      (void)hipGetLastError();
+      throw ApiError(PBF_ERR_INVALID, "dst is not memory HIP knows (allocate it "
+                                  "with hipMalloc or map a registered "
+                                  "graphics buffer)");
+    }
+    need(a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged,
+         "dst is not device memory");
+    need(a.device == S->device, "dst is on another GPU than the simulation");
+    // dst may have been mapped or filled on any stream (the handle's stream
+    // does not wait for the null stream): wait for everything queued on the
+    // GPU first, as the reference finishes GL before it acquires the buffer
+    // (src/Simulation.cpp:464)
+    PBF_HIP(hipDeviceSynchronize());
+    pbf::enqueue_store_positions(S->d, S->scenes[scene],
+                                 static_cast<float4 *>(dst), S->stream);
+    PBF_HIP(hipGetLastError());
+    PBF_HIP(hipStreamSynchronize(S->stream));
+    S->check_err();
+  });
+}
+
 int pbf_set_wave_generator(pbf_sim *S, float value) {
   return guard([&] {
     need(S != nullptr, "sim is NULL");

@@ -1998,6 +1998,27 @@ __global__ __launch_bounds__(256) void k_store_state(
   vel4[id] = vel[k];
 }
 
+// The float4 positions buffer a viewer draws, as the reference's
+// updatePositions leaves it in the buffer it shares with GL
+// (update_positions.cl:9-12): particle p's (x, |v|) at index p.  The scene's
+// particles fill its first n slots (ids id_base .. id_base + n - 1).
+__global__ __launch_bounds__(256) void k_store_positions(
+    const float4 *__restrict__ pos, const int *__restrict__ pid, int n,
+    int id_base, float4 *__restrict__ dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int id = pid[k] - id_base;
+  if ((unsigned)id >= (unsigned)n) return;
+  dst[id] = pos[k];
+}
+
+void enqueue_store_positions(const DevBuffers &d, const Scene &sc,
+                             float4 *dst, hipStream_t st) {
+  if (sc.n <= 0) return;
+  k_store_positions<<<(sc.n + 255) / 256, 256, 0, st>>>(
+      d.pos + sc.slot_off, d.pid + sc.slot_off, sc.n, sc.id_base, dst);
+}
+
 void enqueue_load_state(const DevBuffers &d, const StepShape &sh,
                         const float4 *pos4, const float4 *vel4, int n,
                         hipStream_t st) {

@@ -104,6 +104,10 @@ void enqueue_load_state(const DevBuffers &d, const StepShape &sh,
                         hipStream_t st);
 void enqueue_store_state(const DevBuffers &d, const StepShape &sh,
                          float4 *pos4, float4 *vel4, int n, hipStream_t st);
+// A viewer's positions buffer: (x, |v|) of each particle of scene sc at its
+// index in input order, into dst (device memory, sc.n float4).
+void enqueue_store_positions(const DevBuffers &d, const Scene &sc,
+                             float4 *dst, hipStream_t st);
 // enqueue_predict for counts already zeroed (no memset).
 void enqueue_predict_zeroed(const DevBuffers &d, const StepShape &sh,
                             hipStream_t st);

@@ -114,6 +114,8 @@ _SIGNATURES = [
     ("pbf_dump", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("pbf_batch_dump", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pbf_positions_to_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     ("pbf_set_wave_generator", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
     ("pbf_get_box", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float), _P(ctypes.c_float)]),
     ("pbf_num_particles", ctypes.c_uint32, [ctypes.c_void_p]),
@@ -241,6 +243,13 @@ class Simulation:
 
     def set_wave_generator(self, value: float) -> None:
         _check(self._lib.pbf_set_wave_generator(self._h, value))
+
+    def positions_to_device(self, dst: int, scene: int = 0) -> None:
+        """(x, |v|) of every particle of `scene`, float4 by particle index,
+        written into the device buffer at address `dst` (the positions
+        buffer a viewer draws; a mapped GL buffer or hipMalloc memory)."""
+        _check(self._lib.pbf_positions_to_device(self._h, scene,
+                                                 ctypes.c_void_p(dst)))
 
     def get_sizes_min(self):
         mn = (ctypes.c_float * 4)()

@@ -793,6 +793,69 @@ def test_moving_wall_in_a_batch():
         assert np.array_equal(got[k][1], gv), k
 
 
+def _hip_runtime():
+    """The HIP runtime libpbf.so uses, for device buffers of the test's own."""
+    import ctypes
+    pbf.load_library()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                              ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    for f in ("hipMalloc", "hipMemcpy", "hipMemset", "hipFree"):
+        getattr(hip, f).restype = ctypes.c_int
+    return hip
+
+
+def test_positions_to_device_equals_dump():
+    """The viewer's positions buffer (what the reference's updatePositions
+    leaves in the buffer it shares with GL, update_positions.cl:9-12):
+    pbf_positions_to_device into device memory equals pbf_dump's pos4 bit
+    for bit -- (x, |v|) by particle index -- for a handle and for each scene
+    of a batch; NULL, host memory and a scene out of range are refused
+    before anything is launched."""
+    import ctypes
+    hip = _hip_runtime()
+    sc = scenarios.config(0, physics=pbf.PHYSICS_PAPER)
+    p, parts = sc.params[0], sc.particles[0]
+    n = parts.shape[0]
+    say("positions to device", n)
+    d = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d), n * 16) == 0
+    try:
+        got = np.zeros((n, 4), np.float32)
+        with pbf.Simulation(p, parts) as s:
+            for nsteps in (0, 3):
+                s.step(nsteps)
+                assert hip.hipMemset(d, 0xff, n * 16) == 0
+                s.positions_to_device(d.value)
+                assert hip.hipMemcpy(got.ctypes.data, d, n * 16, 2) == 0
+                pos, _ = s.dump()
+                assert np.array_equal(got, pos), nsteps
+            assert np.all(got[:, 3] > 0.0)          # .w = |v| after steps
+            host = np.zeros((n, 4), np.float32)
+            for bad in (None, host.ctypes.data):
+                with pytest.raises(pbf.PBFError) as e:
+                    s.positions_to_device(bad)
+                assert e.value.code == pbf.ERR_INVALID
+            with pytest.raises(pbf.PBFError) as e:
+                s.positions_to_device(d.value, scene=1)
+            assert e.value.code == pbf.ERR_INVALID
+        c = scenarios.batch_config(nscenes=3, physics=pbf.PHYSICS_PAPER,
+                                   nmin=500, nmax=3000, seed=79)
+        with pbf.BatchSimulation(c.params, c.particles) as b:
+            b.step(4)
+            for k, m in enumerate(c.meta["counts"]):
+                assert hip.hipMemset(d, 0xff, m * 16) == 0
+                b.positions_to_device(d.value, scene=k)
+                part = np.zeros((m, 4), np.float32)
+                assert hip.hipMemcpy(part.ctypes.data, d, m * 16, 2) == 0
+                assert np.array_equal(part, b.dump_scene(k)[0]), k
+    finally:
+        hip.hipFree(d)
+
+
 def test_graph_replays_match_eager_launches():
     sc = scenarios.small_dam(dims=(20, 20, 20), box=(2.0, 2.0, 2.0),
                              physics=pbf.PHYSICS_PAPER, seed=8)
