@@ -31,6 +31,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace pbf {
 
@@ -117,16 +118,32 @@ __device__ __forceinline__ float3 q_diff(unsigned ax, unsigned ay,
   return make_float3((float)dx, (float)dy, (float)dz);
 }
 
+// x, y, z of a float4 record, read by a 12-byte load at the 16-byte stride
+// (the passes that need only a neighbour's position: the search and the
+// lambda replays; the texture path's time grows with the bytes per lane).
+struct Pos3 {
+  float x, y, z;
+};
+
+typedef unsigned U3 __attribute__((ext_vector_type(3)));
+
 // Gathers through a buffer resource: a 32-bit byte offset per lane instead
 // of a 64-bit address, and loads past the n records of the array return 0.
 template <typename T>
 struct Gather {
+  // bytes between records (Pos3 reads float4 records)
+  static constexpr int kStride =
+      std::is_same<T, Pos3>::value ? 16 : (int)sizeof(T);
   __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ Gather(const T *p, int n)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), (short)0,
-                                            n * (int)sizeof(T), 0x00020000)) {}
+  __device__ __forceinline__ Gather(const void *p, int n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
+                                            n * kStride, 0x00020000)) {}
   __device__ __forceinline__ T at_bytes(int off) const {
-    if constexpr (sizeof(T) == 32) {
+    if constexpr (std::is_same<T, Pos3>::value) {
+      const U3 v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+      return Pos3{__uint_as_float(v.x), __uint_as_float(v.y),
+                  __uint_as_float(v.z)};
+    } else if constexpr (sizeof(T) == 32) {
       T t;
       t.x = __builtin_bit_cast(
           float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
@@ -143,7 +160,7 @@ struct Gather {
     }
   }
   __device__ __forceinline__ T operator[](int j) const {
-    return at_bytes(j * (int)sizeof(T));
+    return at_bytes(j * kStride);
   }
 };
 
@@ -844,7 +861,7 @@ struct HitRecorder {
 template <typename R, typename F>
 __device__ __forceinline__ void walk_rows(const Scene &sc,
                                           const int *__restrict__ cs,
-                                          const Gather<float4> &X, float3 xi,
+                                          const Gather<Pos3> &X, float3 xi,
                                           R &&on_row, F &&f) {
   int cx, cy, cz;
   cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
@@ -861,7 +878,7 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
       const int re = (row << kOffBits) - b;
       int j = b;
       for (; j + 4 <= e; j += 4) {
-        const float4 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2], a3 = X[j + 3];
+        const Pos3 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2], a3 = X[j + 3];
         f(j, a0, re);
         f(j + 1, a1, re);
         f(j + 2, a2, re);
@@ -871,7 +888,7 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
         // the 1 to 3 slots left, loaded together (a slot past e is loaded
         // but not visited; past the array a load returns 0) rather than
         // one load in flight at a time
-        const float4 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2];
+        const Pos3 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2];
         f(j, a0, re);
         if (j + 1 < e) f(j + 1, a1, re);
         if (j + 2 < e) f(j + 2, a2, re);
@@ -898,11 +915,12 @@ __device__ __forceinline__ unsigned group_entry(const unsigned (&ws)[4],
 // rb is LDS scratch of 9 x blockDim.x ints.
 template <typename T, typename F>
 __device__ __forceinline__ void for_neighbours(
-    const Scene &sc, const int *__restrict__ cs, const Gather<float4> &XS,
+    const Scene &sc, const int *__restrict__ cs, const Gather<Pos3> &XS,
     const Gather<T> &X, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, const int *__restrict__ rbase, int stride,
     int i, int *rb, F &&f) {
-  constexpr int kShift = sizeof(T) == 32 ? 5 : sizeof(T) == 16 ? 4 : 3;
+  constexpr int kShift =
+      Gather<T>::kStride == 32 ? 5 : Gather<T>::kStride == 16 ? 4 : 3;
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
     // rb[row * bs + lane] = the row's start (lanes of one row in
@@ -937,12 +955,12 @@ __device__ __forceinline__ void for_neighbours(
     }
   } else {
     const float h2 = sc.h2;
-    const float4 s4 = XS[i];
+    const Pos3 s4 = XS[i];
     const float3 xs = make_float3(s4.x, s4.y, s4.z);
     int row = 0;
     walk_rows(
         sc, cs, XS, xs, [&](int r, int, int) { row = r; },
-        [&](int j, float4 sj, int) {
+        [&](int j, Pos3 sj, int) {
           const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
           const float r2 = rx * rx + ry * ry + rz * rz;
           if (in_reach(r2, h2)) f(j, row, X[j]);
@@ -1183,7 +1201,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
   HitRecorder rec{0u, 0u, 0u, 0u, 0, 0, kPerGroup - 1};
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
-  const Gather<float4> xsg(XS, stride);
+  const Gather<Pos3> xsg(XS, stride);
   // the first candidate slot of every stencil row in the grid, for the
   // replaying passes (row 4, the particle's own, also for the padding);
   // rows outside the grid name no entry and get 0
@@ -1198,7 +1216,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
         if (row == 4) self_off = i - b;
         wide = wide || e - b > (1 << kOffBits);
       },
-      [&](int j, float4 xj, int re) {
+      [&](int j, Pos3 xj, int re) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         if (in_reach(r2, h2)) {
@@ -1230,9 +1248,9 @@ __global__ __launch_bounds__(256) void k_lambda_list(
   const float3 xi = make_float3(x4.x, x4.y, x4.z);
   LambdaSums ls;
   const float h = sc.h, h2 = sc.h2;
-  const Gather<float4> xsg(XS, stride), xcg(XC, stride);
+  const Gather<Pos3> xsg(XS, stride), xcg(XC, stride);
   for_neighbours(sc, cs, xsg, xcg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, float4 xj) {
+                 [&](int, int, Pos3 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
     ls.add_if(in_reach(r2, h2), h, h2, rx, ry, rz, r2);
@@ -1265,7 +1283,7 @@ __global__ __launch_bounds__(256) void k_lambda_list_q(
   // which name i itself, then give r = 0 exactly)
   const uint2 qi = QA[i];
   LambdaSums ls;
-  const Gather<float4> xsg(XS, stride);
+  const Gather<Pos3> xsg(XS, stride);
   const Gather<uint2> qag(QA, stride);
   for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
                  [&](int, int, uint2 qj) {
@@ -1300,7 +1318,8 @@ __global__ __launch_bounds__(256) void k_delta_paper(
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
   const int n = sc.scorr_n;
-  const Gather<float4> xsg(XS, stride), bg(B, stride);
+  const Gather<Pos3> xsg(XS, stride);
+  const Gather<float4> bg(B, stride);
   // (lambda_i + lambda_j + s_corr) (h-r)^2 r/|r| summed branch-free (the
   // out-of-reach entries selected to 0).  N4: the paper's exponent n = 4 in
   // every scene (the host picks the instance; the loop over any n costs
@@ -1351,7 +1370,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
 // records hold rho / rho0; om by q_unit^2).
 template <bool QIN, bool OMEGA>
 __device__ __forceinline__ void xsph_sums(
-    const Scene &sc, const int *__restrict__ cs, const Gather<float4> &xsg,
+    const Scene &sc, const int *__restrict__ cs, const Gather<Pos3> &xsg,
     const XVRec *__restrict__ XV, const uint4 *__restrict__ QV,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, int i, int *rb, float3 xi,
@@ -1430,7 +1449,7 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
   const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
   const float4 vi = own.v;
   float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
-  const Gather<float4> xsg(XS, stride);
+  const Gather<Pos3> xsg(XS, stride);
   xsph_sums<QIN, false>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
                         rb_lds, xi, vi, s, om);
   const float c = xsph_factor<QIN>(sc);
@@ -1467,7 +1486,7 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
   const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
   const float4 vi = own.v;
   float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
-  const Gather<float4> xsg(XS, stride);
+  const Gather<Pos3> xsg(XS, stride);
   xsph_sums<QIN, true>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
                        rb_lds, xi, vi, s, om);
   const float cx = xsph_factor<QIN>(sc);
@@ -1517,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_vort_final(
     ey += ry * gs;
     ez += rz * gs;
   };
-  const Gather<float4> xsg(XS, stride);
+  const Gather<Pos3> xsg(XS, stride);
   if constexpr (QIN) {
     // |w_i| rounded as the neighbours' are, so that equal magnitudes cancel
     const uint2 qi = QA[i];
