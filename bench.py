@@ -258,28 +258,33 @@ def rocprof_summary(root: str, particles: int, copy_mib: int,
     for cls, (c, t) in sorted(per.items(), key=lambda kv: -kv[1][1]):
         out.append(f"| {cls} | {c} | {t * 1e3 / c:.2f} |")
     if kt:
-        # launches of each class split where none ran for 5 ms:
-        # the autotuner and warm-up, the graph-launched timed steps, and
-        # their replay with kernel events (the bench's avg_launch_ms)
-        out.append("")
-        out.append("| kernel class | segment | launches | avg us |")
-        out.append("|---|---|---|---|")
+        # run phases (autotuner trials and warm-up, the timed steps, their
+        # replay with kernel events ...) from the class launched once per
+        # step in both physics modes (xsph): a phase ends where no xsph
+        # launch began for 5 ms and ten times its median interval; every
+        # class's launches are then averaged per phase
         rows = kt.execute("select name, start, end from kernels "
                           "order by start").fetchall()
+        anchor = [s0 for name, s0, _ in rows
+                  if kernel_class(_short(name)) == "xsph"]
+        gaps = sorted(b - a for a, b in zip(anchor, anchor[1:]))
+        cut = max(5e6, 10.0 * gaps[len(gaps) // 2]) if gaps else 0.0
+        bounds = [(a + b) / 2 for a, b in zip(anchor, anchor[1:]) if b - a > cut]
+        out.append("")
+        out.append("| kernel class | phase | launches | avg us |")
+        out.append("|---|---|---|---|")
         for cls in ("search", "lambda", "delta", "xsph", "vorticity",
                     "reorder", "scatter", "scan"):
-            d = [(s0, e0) for name, s0, e0 in rows
-                 if kernel_class(_short(name)) == cls]
-            if not d:
-                continue
-            seg = [[d[0]]]
-            for prev, cur in zip(d, d[1:]):
-                if cur[0] - prev[1] > 5e6:
-                    seg.append([])
-                seg[-1].append(cur)
-            for k, sg in enumerate(seg):
-                avg = sum(e0 - s0 for s0, e0 in sg) / len(sg) / 1e3
-                out.append(f"| {cls} | {k} | {len(sg)} | {avg:.2f} |")
+            phase = {}
+            for name, s0, e0 in rows:
+                if kernel_class(_short(name)) != cls:
+                    continue
+                k = sum(1 for b in bounds if s0 > b)
+                c, t = phase.get(k, (0, 0.0))
+                phase[k] = (c + 1, t + (e0 - s0))
+            for k in sorted(phase):
+                c, t = phase[k]
+                out.append(f"| {cls} | {k} | {c} | {t / c / 1e3:.2f} |")
     if traffic_out and fscale and wscale:
         per_cls = {}
         for k in times:
@@ -293,6 +298,62 @@ def rocprof_summary(root: str, particles: int, copy_mib: int,
             json.dump({"particles": particles,
                        "bytes_per_launch": {c: tot / n for c, (n, tot) in per_cls.items()}},
                       f, indent=1)
+    return "\n".join(out)
+
+
+def rocprof_compare(roots, particles: int) -> str:
+    """Markdown table of the neighbour-pass kernels of several profiled runs
+    (each root holding kt, valu and ta sub-directories of rocprofv3
+    databases: a kernel trace, the SQ counters and the texture-path
+    counters): launches and average duration, VALU and VMEM instructions per
+    particle, the share of wave cycles issuing VALU, TA busy cycles and the
+    L1 hit rate."""
+    import glob
+    import sqlite3
+
+    def db(path):
+        files = glob.glob(os.path.join(path, "*.db"))
+        return sqlite3.connect(files[0]) if files else None
+
+    def counters(con):
+        res = {}
+        if con is None:
+            return res
+        for name, cname, v in con.execute(
+                "select kernel_name, counter_name, avg(v) from (select "
+                "kernel_name, counter_name, dispatch_id, sum(value) as v from "
+                "counters_collection group by dispatch_id, counter_name) "
+                "group by kernel_name, counter_name"):
+            res.setdefault(_short(name), {})[cname] = v
+        return res
+
+    out = ["| run | kernel | calls | avg us | VALU instr/particle | "
+           "VMEM rd instr/particle | VALU active (of wave cycles) | "
+           "TA busy (avg cycles per TA) | L1 hit |",
+           "|---|---|---|---|---|---|---|---|---|"]
+    for root in roots:
+        tag = os.path.basename(os.path.normpath(root))
+        kt = db(os.path.join(root, "kt"))
+        sq = counters(db(os.path.join(root, "valu")))
+        ta = counters(db(os.path.join(root, "ta")))
+        rows = kt.execute("select name, count(*), avg(duration), "
+                          "sum(duration) from kernels group by name "
+                          "order by sum(duration) desc").fetchall()
+        for name, calls, avg, _ in rows:
+            k = _short(name)
+            if kernel_class(k) not in ("search", "lambda", "delta", "xsph",
+                                       "vorticity"):
+                continue
+            c, t = sq.get(k, {}), ta.get(k, {})
+            wc = c.get("SQ_WAVE_CYCLES") or 1.0
+            acc = t.get("TCP_TOTAL_CACHE_ACCESSES_sum") or 1.0
+            out.append(
+                f"| {tag} | {k} | {calls} | {avg / 1e3:.1f} | "
+                f"{c.get('SQ_INSTS_VALU', 0) * 64 / particles:.0f} | "
+                f"{c.get('SQ_INSTS_VMEM_RD', 0) * 64 / particles:.0f} | "
+                f"{c.get('SQ_ACTIVE_INST_VALU', 0) / wc:.3f} | "
+                f"{t.get('TA_BUSY_avr', 0):.3g} | "
+                f"{100 * (1 - t.get('TCP_TCC_READ_REQ_sum', 0) / acc):.0f} % |")
     return "\n".join(out)
 
 
@@ -543,9 +604,16 @@ def main():
     ap.add_argument("--traffic-out", metavar="JSON",
                     help="with --rocprof-summary: write per-class HBM bytes "
                          "per launch")
+    ap.add_argument("--rocprof-compare", nargs="+", metavar="DIR",
+                    help="compare the neighbour-pass kernels of profiled runs "
+                         "(kt, valu, ta databases under each DIR) and exit")
     ap.add_argument("--rocprof-summary", metavar="DIR",
                     help="summarise the rocprofv3 databases under DIR and exit")
     args = ap.parse_args()
+    if args.rocprof_compare:
+        print(rocprof_compare(args.rocprof_compare,
+                              scenarios.config(args.config).n))
+        return
     if args.rocprof_summary:
         n = scenarios.config(args.config).n if args.config != 3 else 0
         print(rocprof_summary(args.rocprof_summary, n, 512, args.traffic_out))
