@@ -174,6 +174,11 @@ int main(int argc, char **argv) {
             std::strstr(pbf_last_error(), "scratch") != nullptr,
         "noalloc: beyond the static scratch rejected");
 
+  // the viewer's positions buffer: a NULL handle is refused before any
+  // device call
+  check(pbf_positions_to_device(nullptr, 0, dummy) == PBF_ERR_INVALID,
+        "positions_to_device: NULL sim rejected");
+
   std::printf("sanitized host checks: %d failure(s)\n", failures);
   return failures ? 1 : 0;
 }
