@@ -139,8 +139,14 @@ def test_ref_two_particles(golden_dir):
         assert np.array_equal(gp, op) and np.array_equal(gv, ov)
 
 
-@pytest.mark.parametrize("name", ["27", "box_small", "dam"])
+@pytest.mark.parametrize("name", ["27", "box_small", "dam", "two_updown",
+                                  "box_big", "drop", "dam_coarse", "dam_miles",
+                                  "dam2"])
 def test_ref_reference_scenarios(golden_dir, name):
+    """The reference's own scenario files, 10 steps, bit-identical to the FP32
+    oracle: boxes with a non-zero lower corner and non-cubic cells (drop),
+    h = 1 m and dt = 0.1 (dam_coarse), rest densities from 9.261 to 998,200
+    and particle masses other than 1 (dam2)."""
     say("ref scenario", name)
     p, _ = pbf.read_par(os.path.join(golden_dir, name + ".par"))
     parts = pbf.read_particles(os.path.join(golden_dir, name + ".in"))
@@ -148,6 +154,29 @@ def test_ref_reference_scenarios(golden_dir, name):
     pos, vel = orc.state_from_particles(parts)
     op, ov = orc.run_f32(p, pos, vel, 10)
     assert np.array_equal(gp, op) and np.array_equal(gv, ov)
+
+
+def test_ref_box_turns_non_finite_where_the_oracle_does(golden_dir):
+    """The reference's box scenario (27 particles) under the reference
+    kernels' arithmetic: a neighbour with no pair in the last lambda pass
+    (rho_j = 0) enters XSPH's unguarded 1/rho_j
+    (apply_vorticity_and_viscosity.cl:61) in step 3.  The GPU is
+    bit-identical to the FP32 oracle for steps 1 and 2 and reports the
+    non-finite state (PBF_ERR_NUMERIC) at step 3, where the oracle's state
+    is non-finite too."""
+    say("ref scenario box")
+    p, _ = pbf.read_par(os.path.join(golden_dir, "box.par"))
+    parts = pbf.read_particles(os.path.join(golden_dir, "box.in"))
+    pos, vel = orc.state_from_particles(parts)
+    for k in (1, 2):
+        gp, gv = run_gpu(p, parts, k)
+        op, ov = orc.run_f32(p, pos, vel, k)
+        assert np.array_equal(gp, op) and np.array_equal(gv, ov), k
+    op, ov = orc.run_f32(p, pos, vel, 3)
+    assert not (np.isfinite(op).all() and np.isfinite(ov).all())
+    with pytest.raises(pbf.PBFError) as e:
+        run_gpu(p, parts, 3)
+    assert e.value.code == pbf.ERR_NUMERIC
 
 
 def test_ref_wave_generator(c0_ref):

@@ -102,6 +102,22 @@ def test_read_particles(golden_dir):
     assert np.array_equal(c[:, :7], ref.astype(np.float32))
 
 
+@pytest.mark.parametrize("name", ["two", "two_updown", "27", "box_small", "box",
+                                  "box_big", "dam", "dam2", "dam_coarse",
+                                  "dam_miles", "drop"])
+def test_reference_scenarios_load(golden_dir, name):
+    """Every scenario file the reference ships with a particle file loads
+    unchanged: the .par names its particle file, and the particles equal
+    the file's numeric columns (m, x, v) row by row."""
+    p, part = pbf.read_par(os.path.join(golden_dir, name + ".par"))
+    assert part == name + ".in"
+    assert p.physics == pbf.PHYSICS_REF and p.time_step_length > 0
+    a = pbf.read_particles(os.path.join(golden_dir, part))
+    ref = np.loadtxt(os.path.join(golden_dir, part), skiprows=1, ndmin=2)
+    assert a.shape[0] == ref.shape[0] > 0
+    assert np.array_equal(a[:, :7], ref[:, :7].astype(np.float32))
+
+
 def test_read_particles_stops_at_bad_mass(tmp_path):
     f = tmp_path / "bad.in"
     f.write_text("4\n1 0 0 0 0 0 0\n1 1 1 1 0 0 0\nxyz 1 1 1\n1 2 2 2 0 0 0\n")
