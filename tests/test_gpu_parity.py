@@ -495,6 +495,32 @@ def test_dense_cell_rank_matches_oracle():
     assert np.array_equal(gp, op) and np.array_equal(gv, ov)
 
 
+def test_dense_cell_in_a_batch():
+    """A batch whose middle scene puts 1,000 particles in one cell (ranked by
+    k_reorder_dense, whose cell ranges carry the scene's slot shift) between
+    two ordinary scenes: every scene bit-identical to its single run, and the
+    dense one to the FP32 oracle, over 3 REF steps."""
+    say("edge: dense cell in a batch")
+    rng = np.random.default_rng(19)
+    dense = np.zeros((1000, 10), np.float32)
+    dense[:, 0] = 1.0
+    dense[:, 1:4] = (0.5 + rng.uniform(0.0, 0.09, (1000, 3))).astype(np.float32)
+    c = scenarios.batch_config(nscenes=2, physics=pbf.PHYSICS_REF, nmin=700,
+                               nmax=1500, seed=81)
+    params = [c.params[0], scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF),
+              c.params[1]]
+    parts = [c.particles[0], dense, c.particles[1]]
+    with pbf.BatchSimulation(params, parts) as b:
+        b.step(3)
+        got = [b.dump_scene(k) for k in range(3)]
+    for k in range(3):
+        gp, gv = run_gpu(params[k], parts[k], 3)
+        assert np.array_equal(got[k][0], gp) and np.array_equal(got[k][1], gv), k
+    pos, vel = orc.state_from_particles(dense)
+    op, ov = orc.run_f32(params[1], pos, vel, 3)
+    assert np.array_equal(got[1][0], op) and np.array_equal(got[1][1], ov)
+
+
 def test_hundred_thousand_particles_in_eight_cells():
     """100,000 particles in a box of 2 x 2 x 2 cells (12,500 per cell): the
     sort's permutation is numpy's stable sort of the ids by cell and the cell
