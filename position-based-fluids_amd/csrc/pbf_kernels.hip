@@ -26,6 +26,7 @@
 //                  omega and the vorticity pass (fused with
 //                  update_positions.cl) replay them.
 #include "pbf_kernels.hpp"
+#include "pbf_refmath.hpp"
 
 #include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
@@ -316,13 +317,14 @@ __device__ __forceinline__ bool ref_lambda_pair(const Scene &sc, float3 xi,
   const float r2 = dist2_exact(xi, xj, rx, ry, rz);
   const bool hit = r2 > 0.0f && r2 < sc.h2;
   if (hit) {
-    const float rl = sqrtf(r2);
-    const float gx = rx / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
-    const float gy = ry / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
-    const float gz = rz / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    float ux, uy, uz;
+    const float rl = ref_unit(rx, ry, rz, r2, sc.h2 <= 0x1p40f, ux, uy, uz);
+    const float gx = ux * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gy = uy * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gz = uz * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float w = sc.poly6 * (sc.h2 - r2) * (sc.h2 - r2) * (sc.h2 - r2);
     dens += w;
-    gsum += sqrtf(gx * gx + gy * gy + gz * gz);
+    gsum += ref_sqrt(gx * gx + gy * gy + gz * gz);
     gix += gx;
     giy += gy;
     giz += gz;
@@ -349,10 +351,12 @@ __device__ __forceinline__ void ref_delta_pair(const Scene &sc, float3 xi,
   float rx, ry, rz;
   const float r2 = dist2_exact(xi, xj, rx, ry, rz);
   if (r2 > 0.0f && r2 < sc.h2) {
-    const float rl = sqrtf(r2);
-    const float gx = -1.0f * rx / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
-    const float gy = -1.0f * ry / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
-    const float gz = -1.0f * rz / rl * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    // -1 * rx / rl: -(rx / rl) (negation is exact)
+    float ux, uy, uz;
+    const float rl = ref_unit(rx, ry, rz, r2, sc.h2 <= 0x1p40f, ux, uy, uz);
+    const float gx = -ux * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gy = -uy * sc.spiky * (sc.h - rl) * (sc.h - rl);
+    const float gz = -uz * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float s = li + lj;
     sx += s * gx;
     sy += s * gy;
@@ -412,7 +416,7 @@ __device__ __forceinline__ void ref_xsph_pair(const Scene &sc, float3 xi,
   const float r2 = dist2_exact(xi, xj, rx, ry, rz);
   if (r2 > 0.0f && r2 < sc.h2) {
     const float w = sc.poly6 * (sc.h2 - r2) * (sc.h2 - r2) * (sc.h2 - r2);
-    const float inv = 1.0f / xj.w;
+    const float inv = ref_recip(xj.w);
     sx += inv * (vj.x - vi.x) * w;
     sy += inv * (vj.y - vi.y) * w;
     sz += inv * (vj.z - vi.z) * w;
