@@ -85,7 +85,13 @@ typedef struct pbf_params {
   int32_t fp32_iterations;/* mixed precision: trailing full-FP32 iterations */
   float sphere[4];        /* obstacle sphere (cx, cy, cz, r); r <= 0: none  */
   int32_t block_size;     /* gather work-group size 64/128/256; 0 = tuned   */
-  int32_t reserved[7];
+  int32_t launch_grid;    /* tile kernels: 0 = one work-group per tile of   */
+                          /* block_size slots; g > 0 (a multiple of 8) =    */
+                          /* persistent, g work-groups walking all tiles;   */
+                          /* -1 = persistent, as many work-groups as the    */
+                          /* device holds at once.  block_size 0 and        */
+                          /* launch_grid 0: both tuned                      */
+  int32_t reserved[6];
 } pbf_params;
 
 /* One input particle, layout of the reference's Particle
@@ -180,9 +186,10 @@ const char *pbf_last_error(void);
  * file ("key value" lines; the reference keys plus the physics keys
  * physics ref|paper, iterations, relaxation, scorr_k, scorr_n, scorr_dq,
  * xsph_c, vorticity_eps, gravity_y, precision fp32|mixed, fp32_iterations,
- * sphere cx cy cz r, block_size).  Missing keys keep the REF defaults of
- * pbf_params_init (zero for the box).  part_input_file is copied into
- * part_file (NUL-terminated, at most cap bytes) when part_file != NULL. */
+ * sphere cx cy cz r, block_size, launch_grid).  Missing keys keep the REF
+ * defaults of pbf_params_init (zero for the box).  part_input_file is
+ * copied into part_file (NUL-terminated, at most cap bytes) when
+ * part_file != NULL. */
 int pbf_read_par(const char *path, pbf_params *out, char *part_file,
                  size_t cap);
 
@@ -275,13 +282,27 @@ int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms);
 
 /* Launch configuration of this handle: cfg[0] = work-group size of the
  * searching lambda pass (PAPER) or of the lambda pass (REF).  With
- * block_size 0 and at least 2^18 slots, every gather kernel class gets its
- * own size, chosen by timing 64, 128 and 256 on the first step (cached per
- * shape, and in the file named by the PBF_TUNE_CACHE environment variable);
- * otherwise all use block_size (default 128).  cfg[1] = particles per
+ * block_size 0, launch_grid 0 and at least 2^18 slots, every kernel class
+ * gets its own size and launch form, chosen by timing 64, 128 and 256 with
+ * one work-group per tile and persistent (resident work-groups) on the
+ * first step (cached per shape, and in the file named by the PBF_TUNE_CACHE
+ * environment variable); otherwise all use block_size (default 128) and
+ * launch_grid.  cfg[1] = particles per
  * work-item (1), cfg[2] = sort passes (1: one counting pass), cfg[3] =
  * iterations that gather 16-bit records (mixed precision). */
 int pbf_get_config(const pbf_sim *sim, int32_t cfg[4]);
+
+/* Launch form of each kernel class of this handle, classes in the order of
+ * pbf_profile_steps (predict, scan, scatter, reorder, lambda, delta, xsph,
+ * vorticity, search): block[c] = work-group size, grid[c] = 0 for one
+ * work-group per tile of block[c] slots, the work-group count of a
+ * persistent launch, or -1 for a persistent launch of as many work-groups
+ * as the device holds at once (the autotuner times the per-tile and the
+ * resident persistent form; launch_grid forces one).  Fills min(n, 9) entries; *count (if not NULL) receives 9.  No
+ * reference counterpart (the reference launches every kernel over the
+ * particle count rounded up to 32, Simulation.cpp:138-141). */
+int pbf_get_launch(const pbf_sim *sim, int32_t *block, int32_t *grid,
+                   int32_t n, int32_t *count);
 
 #ifdef __cplusplus
 }

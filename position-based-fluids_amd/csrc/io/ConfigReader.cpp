@@ -110,6 +110,8 @@ ConfigParameters ConfigReader::read(const std::string &filename) const {
         for (int a = 0; a < 4; ++a) ss >> p.physics.sphere[a];
       } else if (key == "block_size") {
         ss >> p.physics.block_size;
+      } else if (key == "launch_grid") {
+        ss >> p.physics.launch_grid;
       } else {
         std::cerr << "Unknown parameter " << key << std::endl
                   << "Leaving it out." << std::endl;

@@ -112,6 +112,10 @@ void validate(const pbf_params &p) {
   if (p.block_size != 0 && p.block_size != 64 && p.block_size != 128 &&
       p.block_size != 256)
     invalid("block_size must be 0, 64, 128 or 256");
+  if (p.launch_grid != -1 &&
+      (p.launch_grid < 0 || p.launch_grid % 8 != 0 ||
+       p.launch_grid > (1 << 20)))
+    invalid("launch_grid must be -1, 0 or a multiple of 8 up to 2^20");
 }
 
 pbf::Scene make_scene(const pbf_params &p) {
@@ -314,8 +318,10 @@ pbf::StepShape make_shape(const pbf_params &p0, const pbf::Scene *scenes,
   sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
   sh.scorr_n4 = 1;
   for (size_t s = 0; s < nscenes; ++s) sh.scorr_n4 &= scenes[s].scorr_n == 4;
-  for (int c = 0; c < pbf::kNumClasses; ++c)
+  for (int c = 0; c < pbf::kNumClasses; ++c) {
     sh.block[c] = p0.block_size ? p0.block_size : 128;
+    sh.grid[c] = p0.launch_grid;
+  }
   return sh;
 }
 
@@ -414,6 +420,7 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
     S.scenes.push_back(sc);
   }
   S.total = (int)ids;
+  for (pbf::Scene &sc : S.scenes) sc.slots_all = (int)slot;
   // PAPER gathers address arrays of up to 32-byte records with 32-bit byte
   // offsets
   if (params[0].physics == PBF_PHYSICS_PAPER && slot > INT32_MAX / 32)
@@ -421,7 +428,9 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   const pbf_params &p0 = params[0];
   pbf::StepShape &sh = S.shape;
   sh = make_shape(p0, S.scenes.data(), S.scenes.size(), params, slot, cells);
-  S.tune_pending = p0.block_size == 0;
+  PBF_HIP(hipDeviceGetAttribute(&sh.cus, hipDeviceAttributeMultiprocessorCount,
+                                S.device));
+  S.tune_pending = p0.block_size == 0 && p0.launch_grid == 0;
 
   const size_t P = (size_t)sh.slots;
   pbf::DevBuffers &d = S.d;
@@ -466,19 +475,22 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
 }
 
 // ---- autotuner ---------------------------------------------------------
-// Chooses the work-group size (64, 128 or 256) of each gather kernel class
-// by timing each class over two steps at every size, on the handle's own
-// state (saved before and restored after); the classes are independent
-// launches, so the fastest size of each is taken.  Results are kept per
-// problem shape for the process and, when PBF_TUNE_CACHE names a file, in
-// that file ("key size size ..." lines, one size per class).  Handles below
-// kTuneMinSlots slots keep the default: there the launch overhead, not the
-// kernels, dominates.
+// Chooses the launch of each kernel class: work-group size 64, 128 or 256,
+// each with one work-group per tile or persistent (as many work-groups as
+// the device holds at once, walking all tiles), by timing each class over two
+// steps in every combination on the handle's own state (saved before and
+// restored after); the classes are independent launches, so the fastest
+// combination of each is taken.  Results are kept per problem shape for the
+// process and, when PBF_TUNE_CACHE names a file, in that file ("key b g b g
+// ..." lines, size and grid per class).  Handles below kTuneMinSlots slots
+// keep the default: there the launch overhead, not the kernels, dominates.
 constexpr int kTuneMinSlots = 1 << 18;
-using Blocks = std::array<int, pbf::kNumClasses>;
+struct Launch {
+  std::array<int, pbf::kNumClasses> block, grid;
+};
 
 std::mutex g_tune_mu;
-std::map<std::string, Blocks> g_tune;
+std::map<std::string, Launch> g_tune;
 
 std::string tune_key(const pbf_sim &S) {
   hipDeviceProp_t prop;
@@ -489,13 +501,16 @@ std::string tune_key(const pbf_sim &S) {
   k << arch << ":" << S.shape.slots << ":" << S.shape.cells << ":"
     << S.shape.physics << ":" << S.shape.iterations << ":"
     << S.shape.fp16_iterations << ":" << S.shape.vorticity << ":"
-    << S.scenes.size() << ":c" << pbf::kNumClasses;
+    << S.scenes.size() << ":c" << pbf::kNumClasses << ":t2";
   return k.str();
 }
 
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256; }
+bool valid_grid(int g) {
+  return g == pbf::kGridResident || (g >= 0 && g % 8 == 0 && g <= (1 << 20));
+}
 
-bool tune_lookup(const std::string &key, Blocks &out) {
+bool tune_lookup(const std::string &key, Launch &out) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
@@ -508,13 +523,15 @@ bool tune_lookup(const std::string &key, Blocks &out) {
     while (std::getline(f, line)) {
       std::istringstream ls(line);
       std::string k;
-      Blocks b;
+      Launch l;
       if (!(ls >> k) || k != key) continue;
       bool ok = true;
-      for (int &v : b) ok = ok && (ls >> v) && valid_block(v);
+      for (int c = 0; c < pbf::kNumClasses; ++c)
+        ok = ok && (ls >> l.block[c] >> l.grid[c]) && valid_block(l.block[c]) &&
+             valid_grid(l.grid[c]);
       if (ok) {
-        g_tune[key] = b;
-        out = b;
+        g_tune[key] = l;
+        out = l;
         return true;
       }
     }
@@ -522,15 +539,21 @@ bool tune_lookup(const std::string &key, Blocks &out) {
   return false;
 }
 
-void tune_store(const std::string &key, const Blocks &b) {
+void tune_store(const std::string &key, const Launch &l) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
-  g_tune[key] = b;
+  g_tune[key] = l;
   if (const char *path = std::getenv("PBF_TUNE_CACHE")) {
     std::ofstream f(path, std::ios::app);
     f << key;
-    for (int v : b) f << " " << v;
+    for (int c = 0; c < pbf::kNumClasses; ++c)
+      f << " " << l.block[c] << " " << l.grid[c];
     f << "\n";
   }
+}
+
+void apply_launch(pbf::StepShape &sh, const Launch &l) {
+  std::copy(l.block.begin(), l.block.end(), sh.block);
+  std::copy(l.grid.begin(), l.grid.end(), sh.grid);
 }
 
 void autotune(pbf_sim &S) {
@@ -538,9 +561,9 @@ void autotune(pbf_sim &S) {
   S.tune_pending = false;
   if (S.shape.slots < kTuneMinSlots) return;
   const std::string key = tune_key(S);
-  Blocks cached;
+  Launch cached;
   if (tune_lookup(key, cached)) {
-    std::copy(cached.begin(), cached.end(), S.shape.block);
+    apply_launch(S.shape, cached);
     return;
   }
   const size_t P = (size_t)S.shape.slots;
@@ -561,32 +584,41 @@ void autotune(pbf_sim &S) {
     copy(spos, S.d.pos, P * sizeof(float4));
     copy(svel, S.d.vel, P * sizeof(float4));
     copy(spid, S.d.pid, P * sizeof(int));
-    Blocks best_b;
+    Launch best;
+    best.block.fill(S.shape.block[0]);
+    best.grid.fill(0);
     std::array<double, pbf::kNumClasses> best_ms;
-    best_b.fill(S.shape.block[0]);
     best_ms.fill(1e30);
     for (int b : {64, 128, 256}) {
-      for (int c = 0; c < pbf::kNumClasses; ++c) S.shape.block[c] = b;
-      copy(S.d.pos, spos, P * sizeof(float4));
-      copy(S.d.vel, svel, P * sizeof(float4));
-      copy(S.d.pid, spid, P * sizeof(int));
-      pbf::enqueue_predict(S.d, S.shape, S.stream);
-      pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);  // warm-up
-      EventTimer timer;
-      for (int t = 0; t < 2; ++t)
-        pbf::enqueue_step(S.d, S.shape, S.stream, &timer);
-      PBF_HIP(hipStreamSynchronize(S.stream));
-      std::array<double, pbf::kNumClasses> ms{};
-      for (const auto &r : timer.recs) {
-        float t = 0.0f;
-        PBF_HIP(hipEventElapsedTime(&t, r.a, r.b));
-        ms[r.cls] += t;
-      }
-      for (int c = 0; c < pbf::kNumClasses; ++c)
-        if (ms[c] < best_ms[c]) {
-          best_ms[c] = ms[c];
-          best_b[c] = b;
+      // one work-group per tile, then persistent with the work-groups the
+      // device holds at once
+      for (int g : {0, pbf::kGridResident}) {
+        for (int c = 0; c < pbf::kNumClasses; ++c) {
+          S.shape.block[c] = b;
+          S.shape.grid[c] = g;
         }
+        copy(S.d.pos, spos, P * sizeof(float4));
+        copy(S.d.vel, svel, P * sizeof(float4));
+        copy(S.d.pid, spid, P * sizeof(int));
+        pbf::enqueue_predict(S.d, S.shape, S.stream);
+        pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);  // warm-up
+        EventTimer timer;
+        for (int t = 0; t < 2; ++t)
+          pbf::enqueue_step(S.d, S.shape, S.stream, &timer);
+        PBF_HIP(hipStreamSynchronize(S.stream));
+        std::array<double, pbf::kNumClasses> ms{};
+        for (const auto &r : timer.recs) {
+          float t = 0.0f;
+          PBF_HIP(hipEventElapsedTime(&t, r.a, r.b));
+          ms[r.cls] += t;
+        }
+        for (int c = 0; c < pbf::kNumClasses; ++c)
+          if (ms[c] < best_ms[c]) {
+            best_ms[c] = ms[c];
+            best.block[c] = b;
+            best.grid[c] = g;
+          }
+      }
     }
     copy(S.d.pos, spos, P * sizeof(float4));
     copy(S.d.vel, svel, P * sizeof(float4));
@@ -594,8 +626,8 @@ void autotune(pbf_sim &S) {
     S.predicted = false;  // the trial steps left their own
     S.ensure_predicted();
     PBF_HIP(hipStreamSynchronize(S.stream));
-    std::copy(best_b.begin(), best_b.end(), S.shape.block);
-    tune_store(key, best_b);
+    apply_launch(S.shape, best);
+    tune_store(key, best);
   } catch (...) {
     S.predicted = false;
     release();
@@ -1037,7 +1069,12 @@ int pbf_step_noalloc(const pbf_params *params, float *pos4, float *vel4,
         ((long long)n + pbf::kSlotAlign - 1) / pbf::kSlotAlign *
         pbf::kSlotAlign;
     if (slots > (1LL << 30)) invalid("problem too large (slots beyond 2^30)");
+    sc.slots_all = (int)slots;
     pbf::StepShape sh = make_shape(*params, &sc, 1, params, slots, sc.ncells);
+    int dev = 0;
+    PBF_HIP(hipGetDevice(&dev));
+    PBF_HIP(hipDeviceGetAttribute(
+        &sh.cus, hipDeviceAttributeMultiprocessorCount, dev));
     // the buffers of the step, 256-byte aligned, from the start of the
     // scratch: first their extent, then their places
     size_t need_bytes = 0;
@@ -1111,6 +1148,19 @@ int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms) {
       throw;
     }
     cleanup();
+  });
+}
+
+int pbf_get_launch(const pbf_sim *S, int32_t *block, int32_t *grid,
+                   int32_t n, int32_t *count) {
+  return guard([&] {
+    need(S != nullptr, "sim is NULL");
+    need(n >= 0 && (n == 0 || (block && grid)), "bad argument");
+    for (int c = 0; c < n && c < pbf::kNumClasses; ++c) {
+      block[c] = S->shape.block[c];
+      grid[c] = S->shape.grid[c];
+    }
+    if (count) *count = pbf::kNumClasses;
   });
 }
 

@@ -44,12 +44,37 @@ namespace pbf {
 
 // Work-groups are dispatched round-robin over the 8 XCDs.  Remap so that
 // each XCD walks one contiguous chunk of slots: spatially adjacent
-// particles (which share stencil rows) then hit the same XCD's L2.
-__device__ __forceinline__ int logical_block() {
-  const int b = blockIdx.x, nb = gridDim.x;
-  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+// particles (which share stencil rows) then hit the same XCD's L2.  t is a
+// position in dispatch order among nt tiles; t & 7 is its XCD, which gets
+// the (t & 7)-th of 8 contiguous chunks of the nt tiles.
+__device__ __forceinline__ int tile_of(int t, int nt) {
+  const int q = nt >> 3, r = nt & 7, x = t & 7, k = t >> 3;
   return x * q + min(x, r) + k;
 }
+
+__device__ __forceinline__ int logical_block() {
+  return tile_of(blockIdx.x, gridDim.x);
+}
+
+// The tiles of a launch (a tile: the blockDim.x slots a work-group
+// covers), for (Tiles<PT> T(scenes); T.more(); T.next()).  PT false: one
+// tile per work-group, gridDim.x tiles.  PT true (persistent launch):
+// gridDim.x work-groups walk all the tiles of the handle's slots,
+// work-group b the positions b, b + gridDim.x, ...; with gridDim.x a
+// multiple of 8 those positions share b's XCD, so each work-group stays
+// within its XCD's chunk.  Each position maps to one tile.
+template <bool PT>
+struct Tiles {
+  int t, nt;
+  __device__ __forceinline__ explicit Tiles(const Scene *__restrict__ scenes)
+      : t(PT ? (int)blockIdx.x : 0),
+        nt(PT ? scenes[0].slots_all / (int)blockDim.x : 1) {}
+  __device__ __forceinline__ bool more() const { return t < nt; }
+  __device__ __forceinline__ int tile() const {
+    return PT ? tile_of(t, nt) : logical_block();
+  }
+  __device__ __forceinline__ void next() { t += PT ? (int)gridDim.x : 1; }
+};
 
 __device__ __forceinline__ int scene_of(const int *__restrict__ block_scene,
                                         int lb) {
@@ -643,33 +668,37 @@ __global__ __launch_bounds__(256) void k_scan_down_sum(
 
 // Also zeroes the count of every occupied cell for the next step's count
 // (the scan has read them; cells without a particle hold 0 already).
+template <bool PT>
 __global__ __launch_bounds__(256) void k_scatter(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const int *__restrict__ key, const int *__restrict__ rank,
     const int *__restrict__ pid, const int *__restrict__ cell_start,
     int *__restrict__ count, int *__restrict__ perm, int *__restrict__ ppid,
     int *__restrict__ err) {
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const int k = key[i];
-  count[k] = 0;
-  const int slot = cell_start[k] + rank[i] + sc.shift;
-  // counts that disagree with the keys would place a slot outside its
-  // scene: reported (error bit 2), never written
-  if ((unsigned)(slot - sc.slot_off) >= (unsigned)sc.n) {
-    atomicOr(err, 2);
-    return;
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const int k = key[i];
+    count[k] = 0;
+    const int slot = cell_start[k] + rank[i] + sc.shift;
+    // counts that disagree with the keys would place a slot outside its
+    // scene: reported (error bit 2), never written
+    if ((unsigned)(slot - sc.slot_off) >= (unsigned)sc.n) {
+      atomicOr(err, 2);
+      continue;
+    }
+    perm[slot] = i;
+    ppid[slot] = pid[i];
   }
-  perm[slot] = i;
-  ppid[slot] = pid[i];
 }
 
 // Arrival order within a cell is not deterministic; rank each slot among
 // its cell by particle id and move it there, gathering x*, x and the id.
 // A cell of more than kDenseCell slots is listed in dense (count, cursor,
 // then (first slot, end) pairs) and left to k_reorder_dense.
+template <bool PT>
 __global__ __launch_bounds__(256) void k_reorder(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const int *__restrict__ perm, const int *__restrict__ ppid,
@@ -677,33 +706,35 @@ __global__ __launch_bounds__(256) void k_reorder(
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
     int *__restrict__ pid_s, float4 *__restrict__ xa,
     float4 *__restrict__ xold, int *__restrict__ dense) {
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int t = lb * blockDim.x + threadIdx.x;
-  if (t - sc.slot_off >= sc.n) return;
-  const int src = perm[t];
-  const int p = ppid[t];
-  const int c = key[src];
-  // clamped to the scene (only counts that disagree with the keys, which
-  // k_scatter reports, reach past it)
-  const int lo = max(cell_start[c] + sc.shift, sc.slot_off);
-  const int hi = min(cell_start[c + 1] + sc.shift, sc.slot_off + sc.n);
-  if (hi - lo > kDenseCell) {
-    // ranked by k_reorder_dense; the cell's first slot lists it
-    if (t == lo) {
-      const int k = atomicAdd(&dense[0], 1);
-      reinterpret_cast<int2 *>(dense + 2)[k] = make_int2(lo, hi);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int t = lb * blockDim.x + threadIdx.x;
+    if (t - sc.slot_off >= sc.n) continue;
+    const int src = perm[t];
+    const int p = ppid[t];
+    const int c = key[src];
+    // clamped to the scene (only counts that disagree with the keys, which
+    // k_scatter reports, reach past it)
+    const int lo = max(cell_start[c] + sc.shift, sc.slot_off);
+    const int hi = min(cell_start[c + 1] + sc.shift, sc.slot_off + sc.n);
+    if (hi - lo > kDenseCell) {
+      // ranked by k_reorder_dense; the cell's first slot lists it
+      if (t == lo) {
+        const int k = atomicAdd(&dense[0], 1);
+        reinterpret_cast<int2 *>(dense + 2)[k] = make_int2(lo, hi);
+      }
+      continue;
     }
-    return;
+    int r = 0;
+    for (int u = lo; u < hi; ++u) r += (ppid[u] < p) ? 1 : 0;
+    const int f = lo + r;
+    const float4 pu = pred_u[src];
+    const float4 x = pos[src];
+    pid_s[f] = p;
+    xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+    xold[f] = make_float4(x.x, x.y, x.z, pu.w);
   }
-  int r = 0;
-  for (int u = lo; u < hi; ++u) r += (ppid[u] < p) ? 1 : 0;
-  const int f = lo + r;
-  const float4 pu = pred_u[src];
-  const float4 x = pos[src];
-  pid_s[f] = p;
-  xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
-  xold[f] = make_float4(x.x, x.y, x.z, pu.w);
 }
 
 // The rank of each slot of a dense cell among the cell by particle id, in
@@ -1014,53 +1045,58 @@ __device__ __forceinline__ void for_stencil_ref_queued(
 }
 
 // compute_scaling.cl:12-140.
+template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
     float *__restrict__ lam, float *__restrict__ rho) {
   extern __shared__ int rq_lds[];
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = X[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
-  for_stencil_ref_queued(sc, cs, X, xi, i, rq_lds, [&](int j) {
-    ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
-  });
-  lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
-  rho[i] = dens;
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = X[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
+    for_stencil_ref_queued(sc, cs, X, xi, i, rq_lds, [&](int j) {
+      ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
+    });
+    lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
+    rho[i] = dens;
+  }
 }
 
 // compute_delta.cl:13-152 + update_predicted.cl.  LAST also forms
 // v = (x* - x)/dt (update_velocities.cl:8) and carries rho of the last
 // lambda pass in .w for the XSPH pass.
-template <bool LAST>
+template <bool PT, bool LAST>
 __global__ __launch_bounds__(256) void k_delta_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
     const float *__restrict__ lam, const float *__restrict__ rho,
     const float4 *__restrict__ xold, float4 *__restrict__ Xn,
     float4 *__restrict__ V) {
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = X[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  const float li = lam[i];
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-    if (j != i) ref_delta_pair(sc, xi, X[j], li, lam[j], sx, sy, sz);
-  });
-  const float3 dp = make_float3(sx / sc.rho0, sy / sc.rho0, sz / sc.rho0);
-  const float3 xn = clamp_update(sc, xi, dp, false);
-  Xn[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
-  if constexpr (LAST) {
-    const float4 xo = xold[i];
-    const float3 v = velocity_exact(sc, xn, xo);
-    V[i] = make_float4(v.x, v.y, v.z, xo.w);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = X[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    const float li = lam[i];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      if (j != i) ref_delta_pair(sc, xi, X[j], li, lam[j], sx, sy, sz);
+    });
+    const float3 dp = make_float3(sx / sc.rho0, sy / sc.rho0, sz / sc.rho0);
+    const float3 xn = clamp_update(sc, xi, dp, false);
+    Xn[i] = make_float4(xn.x, xn.y, xn.z, LAST ? rho[i] : 0.0f);
+    if constexpr (LAST) {
+      const float4 xo = xold[i];
+      const float3 v = velocity_exact(sc, xn, xo);
+      V[i] = make_float4(v.x, v.y, v.z, xo.w);
+    }
   }
 }
 
@@ -1092,29 +1128,32 @@ __device__ __forceinline__ void write_final(int i, float3 xs, float4 v,
 }
 
 // apply_vorticity_and_viscosity.cl:12-104 fused with update_positions.cl.
+template <bool PT>
 __global__ __launch_bounds__(256) void k_xsph_final_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
     const float4 *__restrict__ V, const int *__restrict__ pid_s,
     float4 *__restrict__ pos, float4 *__restrict__ vel,
     int *__restrict__ pid, int *__restrict__ err, NextSort ns) {
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = X[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  const float4 vi = V[i];
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-    if (j != i) ref_xsph_pair(sc, xi, X[j], vi, V[j], sx, sy, sz);
-  });
-  const float4 dv = mul_exact(sc.c_xsph, sx, sy, sz);
-  float len;
-  const float4 v = add_len_exact(vi, dv.x, dv.y, dv.z, len);
-  write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
-  predict_count_slot<kRef>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len), v,
-                           1, ns.pred_u, ns.key, ns.rank, ns.count);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = X[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    const float4 vi = V[i];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
+      if (j != i) ref_xsph_pair(sc, xi, X[j], vi, V[j], sx, sy, sz);
+    });
+    const float4 dv = mul_exact(sc.c_xsph, sx, sy, sz);
+    float len;
+    const float4 v = add_len_exact(vi, dv.x, dv.y, dv.z, len);
+    write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+    predict_count_slot<kRef>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len), v,
+                             1, ns.pred_u, ns.key, ns.rank, ns.count);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1191,51 +1230,55 @@ struct LambdaSums {
 };
 
 // Iteration 0: searches the rows of x*_i, records N_i, and forms lambda.
+template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const int *__restrict__ cs,
     float4 *__restrict__ B, float *__restrict__ rho, uint4 *__restrict__ nl,
     int *__restrict__ ncnt, int *__restrict__ rbase, int stride) {
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = XS[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  HitRecorder rec{0u, 0u, 0u, 0u, 0, 0, kPerGroup - 1};
-  LambdaSums ls;
-  const float h = sc.h, h2 = sc.h2;
-  const Gather<Pos3> xsg(XS, stride);
-  // the first candidate slot of every stencil row in the grid, for the
-  // replaying passes (row 4, the particle's own, also for the padding);
-  // rows outside the grid name no entry and get 0
-  int self_off = -1;
-  unsigned in_grid = 0;
-  bool wide = false;
-  walk_rows(
-      sc, cs, xsg, xi,
-      [&](int row, int b, int e) {
-        rbase[(size_t)row * stride + i] = b;
-        in_grid |= 1u << row;
-        if (row == 4) self_off = i - b;
-        wide = wide || e - b > (1 << kOffBits);
-      },
-      [&](int j, Pos3 xj, int re) {
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (in_reach(r2, h2)) {
-          rec.add((unsigned)(re + j), nl, stride, i);
-          ls.add(h, h2, rx, ry, rz, r2);
-        }
-      });
-#pragma unroll
-  for (int r = 0; r < 9; ++r)
-    if (!(in_grid >> r & 1u)) rbase[(size_t)r * stride + i] = 0;
-  rec.flush(nl, ncnt, stride, i, self_off, wide);
-  ls.store(sc, i, xi, B, rho);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = XS[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    HitRecorder rec{0u, 0u, 0u, 0u, 0, 0, kPerGroup - 1};
+    LambdaSums ls;
+    const float h = sc.h, h2 = sc.h2;
+    const Gather<Pos3> xsg(XS, stride);
+    // the first candidate slot of every stencil row in the grid, for the
+    // replaying passes (row 4, the particle's own, also for the padding);
+    // rows outside the grid name no entry and get 0
+    int self_off = -1;
+    unsigned in_grid = 0;
+    bool wide = false;
+    walk_rows(
+        sc, cs, xsg, xi,
+        [&](int row, int b, int e) {
+          rbase[(size_t)row * stride + i] = b;
+          in_grid |= 1u << row;
+          if (row == 4) self_off = i - b;
+          wide = wide || e - b > (1 << kOffBits);
+        },
+        [&](int j, Pos3 xj, int re) {
+          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          if (in_reach(r2, h2)) {
+            rec.add((unsigned)(re + j), nl, stride, i);
+            ls.add(h, h2, rx, ry, rz, r2);
+          }
+        });
+  #pragma unroll
+    for (int r = 0; r < 9; ++r)
+      if (!(in_grid >> r & 1u)) rbase[(size_t)r * stride + i] = 0;
+    rec.flush(nl, ncnt, stride, i, self_off, wide);
+    ls.store(sc, i, xi, B, rho);
+  }
 }
 
 // Iterations >= 1: lambda over N_i at the positions of the last delta pass.
+template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_list(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ XC,
@@ -1244,22 +1287,24 @@ __global__ __launch_bounds__(256) void k_lambda_list(
     const int *__restrict__ rbase, int stride, float4 *__restrict__ B,
     float *__restrict__ rho) {
   extern __shared__ int rb_lds[];
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = XC[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  LambdaSums ls;
-  const float h = sc.h, h2 = sc.h2;
-  const Gather<Pos3> xsg(XS, stride), xcg(XC, stride);
-  for_neighbours(sc, cs, xsg, xcg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, Pos3 xj) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    ls.add_if(in_reach(r2, h2), h, h2, rx, ry, rz, r2);
-  });
-  ls.store(sc, i, xi, B, rho);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = XC[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    LambdaSums ls;
+    const float h = sc.h, h2 = sc.h2;
+    const Gather<Pos3> xsg(XS, stride), xcg(XC, stride);
+    for_neighbours(sc, cs, xsg, xcg, nl, ncnt, rbase, stride, i, rb_lds,
+                   [&](int, int, Pos3 xj) {
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      ls.add_if(in_reach(r2, h2), h, h2, rx, ry, rz, r2);
+    });
+    ls.store(sc, i, xi, B, rho);
+  }
 }
 
 // The support radius and its square in units of q_unit (exact).
@@ -1269,6 +1314,7 @@ constexpr float kQH2 = kQH * kQH;
 // Mixed precision, iterations 1 .. m - 1: k_lambda_list over 16-bit
 // fixed-point neighbour positions (QA, written by the delta pass before),
 // the sums formed in units of q_unit.
+template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_list_q(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ XC,
@@ -1277,32 +1323,34 @@ __global__ __launch_bounds__(256) void k_lambda_list_q(
     const int *__restrict__ rbase, int stride, float4 *__restrict__ B,
     float *__restrict__ rho) {
   extern __shared__ int rb_lds[];
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = XC[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  // the own record, quantised as the neighbours' are (the padding entries,
-  // which name i itself, then give r = 0 exactly)
-  const uint2 qi = QA[i];
-  LambdaSums ls;
-  const Gather<Pos3> xsg(XS, stride);
-  const Gather<uint2> qag(QA, stride);
-  for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, uint2 qj) {
-    const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
-    const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-    ls.add_if(in_reach(r2, kQH2), kQH, kQH2, r.x, r.y, r.z, r2);
-  });
-  ls.store_q(sc, i, xi, B, rho);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = XC[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    // the own record, quantised as the neighbours' are (the padding entries,
+    // which name i itself, then give r = 0 exactly)
+    const uint2 qi = QA[i];
+    LambdaSums ls;
+    const Gather<Pos3> xsg(XS, stride);
+    const Gather<uint2> qag(QA, stride);
+    for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
+                   [&](int, int, uint2 qj) {
+      const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
+      const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+      ls.add_if(in_reach(r2, kQH2), kQH, kQH2, r.x, r.y, r.z, r2);
+    });
+    ls.store_q(sc, i, xi, B, rho);
+  }
 }
 
 // eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST writes
 // XV = ((x, rho), (v, m)) instead, v = (x - x_n)/dt, for the XSPH pass.
 // WRITE_QA / WRITE_QV (mixed precision) also write the 16-bit records of
 // the next lambda pass / of the XSPH pass.
-template <bool LAST, bool WRITE_QA, bool N4, bool WRITE_QV>
+template <bool PT, bool LAST, bool WRITE_QA, bool N4, bool WRITE_QV>
 __global__ __launch_bounds__(256) void k_delta_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ B,
@@ -1312,57 +1360,59 @@ __global__ __launch_bounds__(256) void k_delta_paper(
     float4 *__restrict__ XC, XVRec *__restrict__ XV, uint2 *__restrict__ QA,
     uint4 *__restrict__ QV) {
   extern __shared__ int rb_lds[];
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = B[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  const float li = x4.w;
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
-  const int n = sc.scorr_n;
-  const Gather<Pos3> xsg(XS, stride);
-  const Gather<float4> bg(B, stride);
-  // (lambda_i + lambda_j + s_corr) (h-r)^2 r/|r| summed branch-free (the
-  // out-of-reach entries selected to 0).  N4: the paper's exponent n = 4 in
-  // every scene (the host picks the instance; the loop over any n costs
-  // registers and occupancy)
-  for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
-                 [&](int, int, float4 xj) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float hr = h - r2 * inv;
-    const float t = h2 - r2;
-    const float q = t * t * t * cq;  // W / W(dq)
-    float qn;
-    if constexpr (N4) {
-      const float q2 = q * q;
-      qn = q2 * q2;
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = B[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    const float li = x4.w;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
+    const int n = sc.scorr_n;
+    const Gather<Pos3> xsg(XS, stride);
+    const Gather<float4> bg(B, stride);
+    // (lambda_i + lambda_j + s_corr) (h-r)^2 r/|r| summed branch-free (the
+    // out-of-reach entries selected to 0).  N4: the paper's exponent n = 4 in
+    // every scene (the host picks the instance; the loop over any n costs
+    // registers and occupancy)
+    for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
+                   [&](int, int, float4 xj) {
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float t = h2 - r2;
+      const float q = t * t * t * cq;  // W / W(dq)
+      float qn;
+      if constexpr (N4) {
+        const float q2 = q * q;
+        qn = q2 * q2;
+      } else {
+        qn = 1.0f;
+        for (int e = 0; e < n; ++e) qn *= q;
+      }
+      const float coef = li + xj.w - kk * qn;
+      const float gs = in_reach(r2, h2) ? coef * hr * hr * inv : 0.0f;
+      ax += rx * gs;
+      ay += ry * gs;
+      az += rz * gs;
+    });
+    // grad_i W = -spiky (h-r)^2 r/|r|
+    const float c = -sc.spiky * sc.inv_rho0;
+    const float3 dp = make_float3(ax * c, ay * c, az * c);
+    const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
+    if constexpr (WRITE_QA) QA[i] = pack_q(sc, xn, 0u);
+    if constexpr (LAST) {
+      const float4 xo = xold[i];
+      const float3 v = velocity_exact(sc, xn, xo);
+      XV[i] = XVRec{make_float4(xn.x, xn.y, xn.z, rho[i]),
+                    make_float4(v.x, v.y, v.z, xo.w)};
+      if constexpr (WRITE_QV) QV[i] = pack_qv(sc, xn, rho[i], v);
     } else {
-      qn = 1.0f;
-      for (int e = 0; e < n; ++e) qn *= q;
+      XC[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
     }
-    const float coef = li + xj.w - kk * qn;
-    const float gs = in_reach(r2, h2) ? coef * hr * hr * inv : 0.0f;
-    ax += rx * gs;
-    ay += ry * gs;
-    az += rz * gs;
-  });
-  // grad_i W = -spiky (h-r)^2 r/|r|
-  const float c = -sc.spiky * sc.inv_rho0;
-  const float3 dp = make_float3(ax * c, ay * c, az * c);
-  const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
-  if constexpr (WRITE_QA) QA[i] = pack_q(sc, xn, 0u);
-  if constexpr (LAST) {
-    const float4 xo = xold[i];
-    const float3 v = velocity_exact(sc, xn, xo);
-    XV[i] = XVRec{make_float4(xn.x, xn.y, xn.z, rho[i]),
-                  make_float4(v.x, v.y, v.z, xo.w)};
-    if constexpr (WRITE_QV) QV[i] = pack_qv(sc, xn, rho[i], v);
-  } else {
-    XC[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
   }
 }
 
@@ -1434,7 +1484,7 @@ __device__ __forceinline__ float xsph_factor(const Scene &sc) {
 
 // XSPH over N_i with 1/rho_j weights fused with update_positions.cl (PAPER
 // without vorticity).
-template <bool QIN>
+template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
@@ -1445,24 +1495,26 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
     float4 *__restrict__ vel, int *__restrict__ pid, int *__restrict__ err,
     NextSort ns) {
   extern __shared__ int rb_lds[];
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const XVRec own = XV[i];
-  const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
-  const float4 vi = own.v;
-  float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
-  const Gather<Pos3> xsg(XS, stride);
-  xsph_sums<QIN, false>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
-                        rb_lds, xi, vi, s, om);
-  const float c = xsph_factor<QIN>(sc);
-  const float4 v =
-      make_float4(vi.x + c * s.x, vi.y + c * s.y, vi.z + c * s.z, vi.w);
-  const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-  write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
-  predict_count_slot<kPaper>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len),
-                             v, 1, ns.pred_u, ns.key, ns.rank, ns.count);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const XVRec own = XV[i];
+    const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
+    const float4 vi = own.v;
+    float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
+    const Gather<Pos3> xsg(XS, stride);
+    xsph_sums<QIN, false>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
+                          rb_lds, xi, vi, s, om);
+    const float c = xsph_factor<QIN>(sc);
+    const float4 v =
+        make_float4(vi.x + c * s.x, vi.y + c * s.y, vi.z + c * s.z, vi.w);
+    const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+    predict_count_slot<kPaper>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len),
+                               v, 1, ns.pred_u, ns.key, ns.rank, ns.count);
+  }
 }
 
 // |omega| as the FP16 payload of the vorticity pass's records (scaled into
@@ -1472,7 +1524,7 @@ constexpr float kOmegaScale = 1.0f / 256.0f;
 // XSPH and vorticity omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15) over
 // N_i; writes B = (x, |omega|) for the vorticity pass, or (QIN) its 16-bit
 // records QA.
-template <bool QIN>
+template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
@@ -1482,27 +1534,30 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
     float4 *__restrict__ OM, float4 *__restrict__ B,
     uint2 *__restrict__ QA) {
   extern __shared__ int rb_lds[];
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const XVRec own = XV[i];
-  const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
-  const float4 vi = own.v;
-  float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
-  const Gather<Pos3> xsg(XS, stride);
-  xsph_sums<QIN, true>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
-                       rb_lds, xi, vi, s, om);
-  const float cx = xsph_factor<QIN>(sc);
-  VX[i] = make_float4(vi.x + cx * s.x, vi.y + cx * s.y, vi.z + cx * s.z, vi.w);
-  const float cw = QIN ? sc.spiky * (sc.q_unit * sc.q_unit) : sc.spiky;
-  const float ox = om.x * cw, oy = om.y * cw, oz = om.z * cw;
-  const float wl = sqrtf(ox * ox + oy * oy + oz * oz);
-  OM[i] = make_float4(ox, oy, oz, wl);
-  if constexpr (QIN)
-    QA[i] = pack_q(sc, xi, f16_bits(wl * kOmegaScale));
-  else
-    B[i] = make_float4(xi.x, xi.y, xi.z, wl);
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const XVRec own = XV[i];
+    const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
+    const float4 vi = own.v;
+    float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
+    const Gather<Pos3> xsg(XS, stride);
+    xsph_sums<QIN, true>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
+                         rb_lds, xi, vi, s, om);
+    const float cx = xsph_factor<QIN>(sc);
+    VX[i] =
+        make_float4(vi.x + cx * s.x, vi.y + cx * s.y, vi.z + cx * s.z, vi.w);
+    const float cw = QIN ? sc.spiky * (sc.q_unit * sc.q_unit) : sc.spiky;
+    const float ox = om.x * cw, oy = om.y * cw, oz = om.z * cw;
+    const float wl = sqrtf(ox * ox + oy * oy + oz * oz);
+    OM[i] = make_float4(ox, oy, oz, wl);
+    if constexpr (QIN)
+      QA[i] = pack_q(sc, xi, f16_bits(wl * kOmegaScale));
+    else
+      B[i] = make_float4(xi.x, xi.y, xi.z, wl);
+  }
 }
 
 // eta = sum (|w_j| - |w_i|) grad_i W_ij over N_i, N = eta/|eta|,
@@ -1510,7 +1565,7 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
 // B = (x, |omega|), or (QIN) the 16-bit records QA (eta formed in units of
 // q_unit and of the FP16 scale: only its direction is used) and the own
 // position from XV.
-template <bool QIN>
+template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_vort_final(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ XS, const float4 *__restrict__ B,
@@ -1522,57 +1577,59 @@ __global__ __launch_bounds__(256) void k_vort_final(
     float4 *__restrict__ pos, float4 *__restrict__ vel, int *__restrict__ pid,
     int *__restrict__ err, NextSort ns) {
   extern __shared__ int rb_lds[];
-  const int lb = logical_block();
-  const Scene &sc = scenes[scene_of(block_scene, lb)];
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i - sc.slot_off >= sc.n) return;
-  const float4 x4 = QIN ? XV[i].x : B[i];
-  const float3 xi = make_float3(x4.x, x4.y, x4.z);
-  const float4 wi = OM[i];
-  float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  const float h = QIN ? kQH : sc.h, h2 = QIN ? kQH2 : sc.h2;
-  auto term = [&](float rx, float ry, float rz, float dw) {
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float hr = h - r2 * inv;
-    const float gs = in_reach(r2, h2) ? dw * hr * hr * inv : 0.0f;
-    ex += rx * gs;
-    ey += ry * gs;
-    ez += rz * gs;
-  };
-  const Gather<Pos3> xsg(XS, stride);
-  if constexpr (QIN) {
-    // |w_i| rounded as the neighbours' are, so that equal magnitudes cancel
-    const uint2 qi = QA[i];
-    const float wq = f16_hi(qi.y);
-    const Gather<uint2> qag(QA, stride);
-    for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
-                   [&](int, int, uint2 qj) {
-      const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
-      term(r.x, r.y, r.z, f16_hi(qj.y) - wq);
-    });
-  } else {
-    const Gather<float4> bg(B, stride);
-    for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
-                   [&](int, int, float4 xj) {
-      term(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, xj.w - wi.w);
-    });
+  for (Tiles<PT> T(scenes); T.more(); T.next()) {
+    const int lb = T.tile();
+    const Scene &sc = scenes[scene_of(block_scene, lb)];
+    const int i = lb * blockDim.x + threadIdx.x;
+    if (i - sc.slot_off >= sc.n) continue;
+    const float4 x4 = QIN ? XV[i].x : B[i];
+    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    const float4 wi = OM[i];
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    const float h = QIN ? kQH : sc.h, h2 = QIN ? kQH2 : sc.h2;
+    auto term = [&](float rx, float ry, float rz, float dw) {
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      const float hr = h - r2 * inv;
+      const float gs = in_reach(r2, h2) ? dw * hr * hr * inv : 0.0f;
+      ex += rx * gs;
+      ey += ry * gs;
+      ez += rz * gs;
+    };
+    const Gather<Pos3> xsg(XS, stride);
+    if constexpr (QIN) {
+      // |w_i| rounded as the neighbours' are, so that equal magnitudes cancel
+      const uint2 qi = QA[i];
+      const float wq = f16_hi(qi.y);
+      const Gather<uint2> qag(QA, stride);
+      for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
+                     [&](int, int, uint2 qj) {
+        const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
+        term(r.x, r.y, r.z, f16_hi(qj.y) - wq);
+      });
+    } else {
+      const Gather<float4> bg(B, stride);
+      for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
+                     [&](int, int, float4 xj) {
+        term(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, xj.w - wi.w);
+      });
+    }
+    float4 v = VX[i];
+    const float e2 = ex * ex + ey * ey + ez * ez;
+    if (e2 > 0.0f) {
+      // the sign of grad_i W (-spiky) flips eta; N = -e/|e|
+      const float inv = -__builtin_amdgcn_rsqf(e2);
+      const float nx = ex * inv, ny = ey * inv, nz = ez * inv;
+      const float c = sc.dt * sc.vort_eps;
+      v.x += c * (ny * wi.z - nz * wi.y);
+      v.y += c * (nz * wi.x - nx * wi.z);
+      v.z += c * (nx * wi.y - ny * wi.x);
+    }
+    const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
+    predict_count_slot<kPaper>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len),
+                               v, 1, ns.pred_u, ns.key, ns.rank, ns.count);
   }
-  float4 v = VX[i];
-  const float e2 = ex * ex + ey * ey + ez * ez;
-  if (e2 > 0.0f) {
-    // the sign of grad_i W (-spiky) flips eta; N = -e/|e|
-    const float inv = -__builtin_amdgcn_rsqf(e2);
-    const float nx = ex * inv, ny = ey * inv, nz = ez * inv;
-    const float c = sc.dt * sc.vort_eps;
-    v.x += c * (ny * wi.z - nz * wi.y);
-    v.y += c * (nz * wi.x - nx * wi.z);
-    v.z += c * (nx * wi.y - ny * wi.x);
-  }
-  const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-  write_final(i, xi, v, len, pos, vel, pid, pid_s, err);
-  predict_count_slot<kPaper>(sc, i, true, make_float4(xi.x, xi.y, xi.z, len),
-                             v, 1, ns.pred_u, ns.key, ns.rank, ns.count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1722,6 +1779,17 @@ struct Launcher {
   KernelTimer *timer;
 };
 
+// Work-groups of block threads (smem bytes of LDS each) the device holds at
+// once for kernel k, a multiple of 8 (one chunk per XCD); 0 if unknown.
+int resident_grid(const void *k, int block, size_t smem, int cus) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, smem) !=
+          hipSuccess ||
+      per_cu <= 0)
+    return 0;
+  return per_cu * cus / 8 * 8;
+}
+
 }  // namespace
 
 // With a timer, the kernel is launched with start/stop events attached to
@@ -1741,13 +1809,25 @@ struct Launcher {
   } while (0)
 #define PBF_LAUNCH(L, cls, kern, grid, block, ...) \
   PBF_LAUNCH_SMEM(L, cls, kern, grid, block, 0, __VA_ARGS__)
-// A per-particle gather kernel at its class's work-group size, with
-// lds_ints ints of LDS per work-item (the replay row bases).
-#define PBF_GATHER(L, cls, kern, lds_ints, ...)                          \
-  PBF_LAUNCH_SMEM(L, cls, kern, (L).sh.slots / (L).sh.block[cls],        \
-                  (L).sh.block[cls],                                     \
-                  (size_t)(lds_ints) * (L).sh.block[cls] * sizeof(int),  \
-                  __VA_ARGS__)
+// A tile kernel (kp: its persistent instance, k1: one work-group per tile)
+// at work-group size bs in its class's launch form, with lds_ints ints of
+// LDS per work-item (the replay row bases).
+#define PBF_TILES(L, cls, bs, kp, k1, lds_ints, ...)                         \
+  do {                                                                       \
+    const int bs_ = (bs), nt_ = (L).sh.slots / bs_;                          \
+    const size_t smem_ = (size_t)(lds_ints) * bs_ * sizeof(int);             \
+    int g_ = (L).sh.grid[cls];                                               \
+    if (g_ == kGridResident)                                                 \
+      g_ = resident_grid(reinterpret_cast<const void *>(kp), bs_, smem_,     \
+                         (L).sh.cus);                                        \
+    if (g_ > 0 && g_ < nt_)                                                  \
+      PBF_LAUNCH_SMEM(L, cls, kp, g_, bs_, smem_, __VA_ARGS__);              \
+    else                                                                     \
+      PBF_LAUNCH_SMEM(L, cls, k1, nt_, bs_, smem_, __VA_ARGS__);             \
+  } while (0)
+// A per-particle gather kernel at its class's work-group size.
+#define PBF_GATHER(L, cls, kp, k1, lds_ints, ...) \
+  PBF_TILES(L, cls, (L).sh.block[cls], kp, k1, lds_ints, __VA_ARGS__)
 
 // The sort of a step.  with_predict: predict, bin and count here (probes,
 // diagnostics); otherwise the previous step's last kernel (or
@@ -1775,12 +1855,12 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
                reinterpret_cast<const int4 *>(d.count), d.partial,
                reinterpret_cast<int4 *>(d.cell_start));
   }
-  PBF_LAUNCH(L, kClsScatter, k_scatter, grid, bs, d.scenes, d.block_scene,
-             d.key, d.rank, d.pid, d.cell_start, d.count, d.perm, d.ppid,
-             d.err);
-  PBF_LAUNCH(L, kClsReorder, k_reorder, grid, bs, d.scenes, d.block_scene,
-             d.perm, d.ppid, d.key, d.cell_start, d.pred_u, d.pos, d.pid_s,
-             d.xa, d.xold, d.dense);
+  PBF_TILES(L, kClsScatter, bs, (k_scatter<true>), (k_scatter<false>), 0,
+            d.scenes, d.block_scene, d.key, d.rank, d.pid, d.cell_start,
+            d.count, d.perm, d.ppid, d.err);
+  PBF_TILES(L, kClsReorder, bs, (k_reorder<true>), (k_reorder<false>), 0,
+            d.scenes, d.block_scene, d.perm, d.ppid, d.key, d.cell_start,
+            d.pred_u, d.pos, d.pid_s, d.xa, d.xold, d.dense);
   PBF_LAUNCH(L, kClsReorder, k_reorder_dense, kDenseBlocks, 1024, d.perm,
              d.ppid, d.pred_u, d.pos, d.pid_s, d.xa, d.xold, d.dense, d.dsc);
 }
@@ -1789,25 +1869,37 @@ static NextSort next_sort(const DevBuffers &d) {
   return NextSort{d.pred_u, d.key, d.rank, d.count};
 }
 
+// A REF delta pass.
+template <bool LAST>
+static void delta_ref_pass(const DevBuffers &d, Launcher &L,
+                           const float4 *cur, float4 *nxt) {
+  PBF_GATHER(L, kClsDelta, (k_delta_ref<true, LAST>),
+             (k_delta_ref<false, LAST>), 0, d.scenes, d.block_scene, cur,
+             d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
+}
+
+static void lambda_ref_pass(const DevBuffers &d, Launcher &L,
+                            const float4 *cur) {
+  PBF_GATHER(L, kClsLambda, (k_lambda_ref<true>), (k_lambda_ref<false>),
+             kRefQueue, d.scenes, d.block_scene, cur, d.cell_start, d.lam,
+             d.rho);
+}
+
 static void step_ref(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
   sort_stage<kRef>(d, L, 1, false);
   float4 *cur = d.xa, *nxt = d.xb;
   for (int it = 0; it < sh.iterations; ++it) {
-    PBF_GATHER(L, kClsLambda, k_lambda_ref, kRefQueue, d.scenes,
-               d.block_scene, cur, d.cell_start, d.lam, d.rho);
-    if (it == sh.iterations - 1) {
-      PBF_GATHER(L, kClsDelta, k_delta_ref<true>, 0, d.scenes, d.block_scene,
-                 cur, d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
-    } else {
-      PBF_GATHER(L, kClsDelta, k_delta_ref<false>, 0, d.scenes, d.block_scene,
-                 cur, d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
-    }
+    lambda_ref_pass(d, L, cur);
+    if (it == sh.iterations - 1)
+      delta_ref_pass<true>(d, L, cur, nxt);
+    else
+      delta_ref_pass<false>(d, L, cur, nxt);
     std::swap(cur, nxt);
   }
-  PBF_GATHER(L, kClsXsph, k_xsph_final_ref, 0, d.scenes, d.block_scene, cur,
-             d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err,
-             next_sort(d));
+  PBF_GATHER(L, kClsXsph, (k_xsph_final_ref<true>), (k_xsph_final_ref<false>),
+             0, d.scenes, d.block_scene, cur, d.cell_start, d.v, d.pid_s,
+             d.pos, d.vel, d.pid, d.err, next_sort(d));
 }
 
 // A delta pass, at the s_corr exponent instance of the step's scenes.
@@ -1815,41 +1907,81 @@ template <bool LAST, bool WRITE_QA, bool WRITE_QV>
 static void delta_pass(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
   if (L.sh.scorr_n4)
-    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_QA, true, WRITE_QV>),
-               9, d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+    PBF_GATHER(L, kClsDelta,
+               (k_delta_paper<true, LAST, WRITE_QA, true, WRITE_QV>),
+               (k_delta_paper<false, LAST, WRITE_QA, true, WRITE_QV>), 9,
+               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
                d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
   else
-    PBF_GATHER(L, kClsDelta, (k_delta_paper<LAST, WRITE_QA, false, WRITE_QV>),
-               9, d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+    PBF_GATHER(L, kClsDelta,
+               (k_delta_paper<true, LAST, WRITE_QA, false, WRITE_QV>),
+               (k_delta_paper<false, LAST, WRITE_QA, false, WRITE_QV>), 9,
+               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
                d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
 }
 
 static void search_pass(const DevBuffers &d, Launcher &L) {
-  PBF_GATHER(L, kClsSearch, k_lambda_paper, 0, d.scenes, d.block_scene, d.xa,
-             d.cell_start, d.xb, d.rho, d.nl, d.ncnt, d.rbase, L.sh.slots);
+  PBF_GATHER(L, kClsSearch, (k_lambda_paper<true>), (k_lambda_paper<false>),
+             0, d.scenes, d.block_scene, d.xa, d.cell_start, d.xb, d.rho,
+             d.nl, d.ncnt, d.rbase, L.sh.slots);
+}
+
+// A lambda pass replaying the lists, over FP32 positions or (Q) over the
+// 16-bit records.
+template <bool Q>
+static void lambda_list_pass(const DevBuffers &d, Launcher &L) {
+  const int P = L.sh.slots;
+  if constexpr (Q)
+    PBF_GATHER(L, kClsLambda, (k_lambda_list_q<true>),
+               (k_lambda_list_q<false>), 9, d.scenes, d.block_scene, d.xa,
+               d.xc, d.qa, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb,
+               d.rho);
+  else
+    PBF_GATHER(L, kClsLambda, (k_lambda_list<true>), (k_lambda_list<false>),
+               9, d.scenes, d.block_scene, d.xa, d.xc, d.cell_start, d.nl,
+               d.ncnt, d.rbase, P, d.xb, d.rho);
+}
+
+// XSPH + omega and the vorticity pass (fused with the position update and
+// the next step's predict), over FP32 records or (Q) the 16-bit ones.
+template <bool Q>
+static void vorticity_passes(const DevBuffers &d, Launcher &L) {
+  const int P = L.sh.slots;
+  PBF_GATHER(L, kClsXsph, (k_xsph_omega<true, Q>), (k_xsph_omega<false, Q>),
+             9, d.scenes, d.block_scene, d.xa, d.xv, d.qv, d.cell_start, d.nl,
+             d.ncnt, d.rbase, P, d.vx, d.om, d.xb, d.qa);
+  PBF_GATHER(L, kClsVort, (k_vort_final<true, Q>), (k_vort_final<false, Q>),
+             9, d.scenes, d.block_scene, d.xa, d.xb, d.xv, d.qa, d.cell_start,
+             d.vx, d.om, d.nl, d.ncnt, d.rbase, P, d.pid_s, d.pos, d.vel,
+             d.pid, d.err, next_sort(d));
+}
+
+// XSPH fused with the position update and the next step's predict (no
+// vorticity confinement).
+template <bool Q>
+static void xsph_final_pass(const DevBuffers &d, Launcher &L) {
+  const int P = L.sh.slots;
+  PBF_GATHER(L, kClsXsph, (k_xsph_final_paper<true, Q>),
+             (k_xsph_final_paper<false, Q>), 9, d.scenes, d.block_scene, d.xa,
+             d.xv, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.pid_s,
+             d.pos, d.vel, d.pid, d.err, next_sort(d));
 }
 
 static void step_paper(const DevBuffers &d, Launcher &L) {
   const StepShape &sh = L.sh;
   sort_stage<kPaper>(d, L, 1, false);
-  const float4 *XS = d.xa;
-  const int P = sh.slots;
   // mixed precision: the lambda passes of iterations 1 .. m - 1 (m =
   // fp16_iterations; iteration 0's is the FP32 search) gather 16-bit
   // positions, and so do XSPH and the vorticity pass; every delta pass
   // gathers FP32 records and updates FP32 positions
   const int m = sh.fp16_iterations;
   for (int it = 0; it < sh.iterations; ++it) {
-    if (it == 0) {
+    if (it == 0)
       search_pass(d, L);
-    } else if (it < m) {
-      PBF_GATHER(L, kClsLambda, k_lambda_list_q, 9, d.scenes, d.block_scene,
-                 XS, d.xc, d.qa, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb,
-                 d.rho);
-    } else {
-      PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
-                 XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
-    }
+    else if (it < m)
+      lambda_list_pass<true>(d, L);
+    else
+      lambda_list_pass<false>(d, L);
     if (it == sh.iterations - 1) {
       if (m > 0)
         delta_pass<true, false, true>(d, L);
@@ -1862,31 +1994,14 @@ static void step_paper(const DevBuffers &d, Launcher &L) {
     }
   }
   if (sh.vorticity) {
-    if (m > 0) {
-      PBF_GATHER(L, kClsXsph, k_xsph_omega<true>, 9, d.scenes, d.block_scene,
-                 XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx,
-                 d.om, d.xb, d.qa);
-      PBF_GATHER(L, kClsVort, k_vort_final<true>, 9, d.scenes, d.block_scene,
-                 XS, d.xb, d.xv, d.qa, d.cell_start, d.vx, d.om, d.nl, d.ncnt,
-                 d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err,
-                 next_sort(d));
-    } else {
-      PBF_GATHER(L, kClsXsph, k_xsph_omega<false>, 9, d.scenes, d.block_scene,
-                 XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx,
-                 d.om, d.xb, d.qa);
-      PBF_GATHER(L, kClsVort, k_vort_final<false>, 9, d.scenes, d.block_scene,
-                 XS, d.xb, d.xv, d.qa, d.cell_start, d.vx, d.om, d.nl, d.ncnt,
-                 d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err,
-                 next_sort(d));
-    }
+    if (m > 0)
+      vorticity_passes<true>(d, L);
+    else
+      vorticity_passes<false>(d, L);
   } else if (m > 0) {
-    PBF_GATHER(L, kClsXsph, k_xsph_final_paper<true>, 9, d.scenes,
-               d.block_scene, XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt,
-               d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err, next_sort(d));
+    xsph_final_pass<true>(d, L);
   } else {
-    PBF_GATHER(L, kClsXsph, k_xsph_final_paper<false>, 9, d.scenes,
-               d.block_scene, XS, d.xv, d.qv, d.cell_start, d.nl, d.ncnt,
-               d.rbase, P, d.pid_s, d.pos, d.vel, d.pid, d.err, next_sort(d));
+    xsph_final_pass<false>(d, L);
   }
 }
 
@@ -1914,23 +2029,17 @@ bool repeatable_class(const StepShape &sh, int cls) {
 bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
                     int cls, int reps) {
   Launcher L{sh, st, nullptr};
-  const int P = sh.slots;
-  const float4 *XS = d.xa;
   if (!repeatable_class(sh, cls)) return false;
   for (int r = 0; r < reps; ++r) {
     if (sh.physics == kRef) {
       if (cls == kClsLambda)
-        PBF_GATHER(L, kClsLambda, k_lambda_ref, kRefQueue, d.scenes,
-                   d.block_scene, d.xa, d.cell_start, d.lam, d.rho);
+        lambda_ref_pass(d, L, d.xa);
       else
-        PBF_GATHER(L, kClsDelta, k_delta_ref<false>, 0, d.scenes,
-                   d.block_scene, d.xa, d.cell_start, d.lam, d.rho, d.xold,
-                   d.xb, d.v);
+        delta_ref_pass<false>(d, L, d.xa, d.xb);
     } else if (cls == kClsSearch) {
       search_pass(d, L);
     } else if (cls == kClsLambda) {
-      PBF_GATHER(L, kClsLambda, k_lambda_list, 9, d.scenes, d.block_scene,
-                 XS, d.xc, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb, d.rho);
+      lambda_list_pass<false>(d, L);
     } else {
       delta_pass<false, false, false>(d, L);
     }
@@ -2063,15 +2172,16 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
     sort_stage<kRef>(d, L, 1, true);
     k_count_nbrs<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                             d.cell_start, d.nbr);
-    k_lambda_ref<<<grid, bs, kRefQueue * bs * sizeof(int), st>>>(
+    k_lambda_ref<false><<<grid, bs, kRefQueue * bs * sizeof(int), st>>>(
         d.scenes, d.block_scene, d.xa, d.cell_start, d.lam, d.rho);
   } else {
     sort_stage<kPaper>(d, L, 1, true);
     k_count_nbrs<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                               d.cell_start, d.nbr);
-    k_lambda_paper<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
-                                        d.cell_start, d.xb, d.rho, d.nl,
-                                        d.ncnt, d.rbase, sh.slots);
+    k_lambda_paper<false><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
+                                               d.cell_start, d.xb, d.rho,
+                                               d.nl, d.ncnt, d.rbase,
+                                               sh.slots);
     k_unpack_lambda<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xb,
                                          d.lam);
   }

@@ -38,6 +38,10 @@ constexpr int kDenseBuckets = 4096;
 // per cell: the 32,768-particle dam break in PAPER physics uses 14 MB).
 constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
 
+// StepShape::grid value of a persistent launch sized to the device's
+// resident work-groups.
+constexpr int kGridResident = -1;
+
 // kernel classes reported by the profiler
 enum KernelClass {
   kClsPredict = 0,
@@ -63,6 +67,12 @@ struct StepShape {
   int vorticity;    // PAPER: vorticity confinement pass enabled
   int scorr_n4;     // every scene has the s_corr exponent n = 4
   int block[kNumClasses];  // work-group size (64, 128 or 256) per class
+  // launch form per class: 0 = one work-group per tile of block[c] slots;
+  // g > 0 = persistent, g work-groups (a multiple of 8) walking the tiles;
+  // kGridResident = persistent with the work-groups the device holds at
+  // once (the kernel's occupancy times the compute units)
+  int grid[kNumClasses];
+  int cus;          // compute units of the device
 };
 
 // Per-kernel-class event timing (pbf_profile_steps, the autotuner): hands

@@ -68,7 +68,8 @@ struct Scene {
   float reach2;        // PAPER: ((1 + kReachMargin) h)^2
   float q_scale;       // mixed precision: 2^kQBits / h
   float q_unit;        // mixed precision: h / 2^kQBits
-  int pad[1];
+  int slots_all;       // padded slots of all scenes of the handle (the
+                       // same in every scene; persistent launches)
 };
 static_assert(sizeof(Scene) % 16 == 0, "Scene must stay 16-byte aligned");
 

@@ -21,6 +21,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpbf.so")
 
+# kernel classes in the library's order (pbf_profile_steps, pbf_get_launch)
+KERNEL_CLASSES = ("predict_count", "scan", "scatter", "reorder", "lambda",
+                  "delta", "xsph", "vorticity", "search")
+
 PHYSICS_REF = 0
 PHYSICS_PAPER = 1
 PRECISION_FP32 = 0
@@ -62,7 +66,8 @@ class Params(ctypes.Structure):
         ("fp32_iterations", ctypes.c_int32),
         ("sphere", ctypes.c_float * 4),
         ("block_size", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("launch_grid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
     def copy(self) -> "Params":
@@ -132,6 +137,9 @@ _SIGNATURES = [
     ("pbf_time_kernel", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32, _P(ctypes.c_double)]),
     ("pbf_get_config", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32)]),
+    ("pbf_get_launch", ctypes.c_int,
+     [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32), ctypes.c_int32,
+      _P(ctypes.c_int32)]),
     ("pbf_calibrate_copy", ctypes.c_int,
      [ctypes.c_uint64, ctypes.c_uint32, _P(ctypes.c_double)]),
     ("pbf_step_noalloc", ctypes.c_int,
@@ -321,6 +329,18 @@ class Simulation:
         cfg = (ctypes.c_int32 * 4)()
         _check(self._lib.pbf_get_config(self._h, cfg))
         return tuple(cfg)
+
+    def launch(self) -> dict:
+        """Launch form per kernel class: {class: (work-group size, grid)},
+        grid 0 for one work-group per tile, else the work-group count of a
+        persistent launch."""
+        block = (ctypes.c_int32 * 16)()
+        grid = (ctypes.c_int32 * 16)()
+        count = ctypes.c_int32()
+        _check(self._lib.pbf_get_launch(self._h, block, grid, 16,
+                                        ctypes.byref(count)))
+        return {KERNEL_CLASSES[c]: (block[c], grid[c])
+                for c in range(count.value)}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -735,12 +735,73 @@ def test_work_group_sizes_give_identical_bits(physics, precision):
         assert np.array_equal(gp, runs[0][0]) and np.array_equal(gv, runs[0][1])
 
 
+@pytest.mark.parametrize("physics, precision", [
+    (pbf.PHYSICS_REF, pbf.PRECISION_FP32),
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_FP32),
+    (pbf.PHYSICS_PAPER, pbf.PRECISION_MIXED)])
+def test_persistent_launches_give_identical_bits(physics, precision):
+    """A persistent launch (launch_grid work-groups walking every tile) runs
+    each tile's work-items exactly as a launch of one work-group per tile:
+    the same bits, for grids that leave each work-group many tiles (8),
+    a few (40: not a divisor of the tile count), at 64 and 256 work-items,
+    and for the grid of work-groups the device holds at once (-1; above
+    this scene's tile count, so its launches fall back to one work-group
+    per tile)."""
+    sc = scenarios.small_dam(dims=(32, 24, 20), box=(2.0, 2.0, 2.0),
+                             physics=physics, seed=43)
+    say("persistent launches", physics, precision, sc.n)
+    assert sc.n // 256 > 40   # every grid below is below the tile count
+    runs = []
+    for bs, grid in ((128, 0), (128, 8), (128, 40), (64, 40), (256, 8),
+                     (128, -1)):
+        p = sc.params[0].copy()
+        p.block_size, p.launch_grid, p.precision = bs, grid, precision
+        with pbf.Simulation(p, sc.particles[0]) as s:
+            launch = s.launch()
+            assert launch["delta"] == (bs, grid)
+            assert launch["scatter"][1] == grid
+            s.step(4)
+            runs.append(s.dump())
+    for gp, gv in runs[1:]:
+        assert np.array_equal(gp, runs[0][0]) and np.array_equal(gv, runs[0][1])
+
+
+def test_persistent_launch_of_an_uneven_batch():
+    """A batch of scenes of uneven sizes (a work-group's tiles then belong
+    to different scenes) stepped with persistent launches equals the same
+    batch with one work-group per tile, and the scenes run one by one."""
+    cfgs = [((4, 5, 6), 1), ((20, 16, 12), 2), ((9, 3, 17), 3), ((12, 12, 12), 4)]
+    scenes = [scenarios.small_dam(dims=d, box=(2.0, 2.0, 2.0),
+                                  physics=pbf.PHYSICS_PAPER, seed=sd)
+              for d, sd in cfgs]
+    say("persistent batch", [s.n for s in scenes])
+    out = []
+    for grid in (0, 16):
+        ps = []
+        for sc in scenes:
+            p = sc.params[0].copy()
+            p.block_size, p.launch_grid = 128, grid
+            ps.append(p)
+        with pbf.BatchSimulation(ps, [sc.particles[0] for sc in scenes]) as b:
+            b.step(3)
+            out.append([b.dump_scene(q) for q in range(len(scenes))])
+    for q, sc in enumerate(scenes):
+        for k in range(2):
+            assert np.array_equal(out[0][q][k], out[1][q][k]), (q, k)
+        p = sc.params[0].copy()
+        p.block_size = 128
+        gp, gv = run_gpu(p, sc.particles[0], 3)
+        assert np.array_equal(out[1][q][0], gp) and np.array_equal(out[1][q][1], gv)
+
+
 def test_autotuner_restores_state_and_writes_cache(tmp_path, monkeypatch):
     """With block_size 0 and at least 2^18 slots the first step times every
-    kernel class at 64, 128 and 256 work-items on the handle's own state.
-    The state is restored afterwards: the tuned run equals a run at a fixed
-    size bit for bit (sizes change no bit).  The choice is appended to the
-    file PBF_TUNE_CACHE names, one line per problem shape."""
+    kernel class at 64, 128 and 256 work-items, each with one work-group
+    per tile and persistent, on the handle's own state.  The state is
+    restored afterwards: the tuned run equals a run at a fixed size bit for
+    bit (sizes and launch forms change no bit).  The choice is appended to
+    the file PBF_TUNE_CACHE names, one line per problem shape: size and
+    grid per kernel class."""
     cache = tmp_path / "tune.txt"
     monkeypatch.setenv("PBF_TUNE_CACHE", str(cache))
     sc = scenarios.small_dam(dims=(64, 64, 64), box=(4.0, 4.0, 4.0),
@@ -761,7 +822,15 @@ def test_autotuner_restores_state_and_writes_cache(tmp_path, monkeypatch):
     assert len([l for l in lines if l]) == 1
     fields = lines[0].split()
     assert fields[0].startswith("gfx")
-    assert len(fields) > 2 and all(f in ("64", "128", "256") for f in fields[1:])
+    vals = [int(f) for f in fields[1:]]
+    assert len(vals) == 2 * len(pbf.KERNEL_CLASSES)
+    assert all(b in (64, 128, 256) for b in vals[0::2])
+    assert all(g in (0, -1) for g in vals[1::2])   # per tile or resident
+    # a new handle of the same shape takes the cached choice
+    with pbf.Simulation(sc.params[0], sc.particles[0]) as s:
+        s.step(1)
+        got = s.launch()
+    assert [got[c] for c in pbf.KERNEL_CLASSES] == list(zip(vals[0::2], vals[1::2]))
 
 
 @pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])

@@ -58,10 +58,12 @@ def test_read_par_matches_reference_file(golden_dir):
 def test_read_par_physics_keys_and_unknown_key(tmp_path):
     f = tmp_path / "s.par"
     f.write_text("part_input_file a.in\nx_max 2\nx_n 20\nphysics paper\n"
-                 "iterations 3\nscorr_k 0.002\nsphere 1 2 3 0.5\nbogus 1\n")
+                 "iterations 3\nscorr_k 0.002\nsphere 1 2 3 0.5\nbogus 1\n"
+                 "launch_grid 2048\n")
     p, part = pbf.read_par(str(f))
     assert part == "a.in" and p.physics == pbf.PHYSICS_PAPER
     assert p.iterations == 3 and p.scorr_k == np.float32(0.002)
+    assert p.launch_grid == 2048
     assert list(p.sphere) == [1.0, 2.0, 3.0, 0.5]
     assert p.x_max == 2.0 and p.x_n == 20.0
 
@@ -177,6 +179,13 @@ def test_create_rejects_bad_params():
         with pytest.raises(pbf.PBFError) as e:
             pbf.Simulation(bad, parts)
         assert "fp32_iterations" in str(e.value)
+    # a persistent grid is a multiple of 8 work-groups (one XCD chunk each)
+    for grid in (-8, 12, (1 << 20) + 8):
+        bad = p.copy()
+        bad.launch_grid = grid
+        with pytest.raises(pbf.PBFError) as e:
+            pbf.Simulation(bad, parts)
+        assert e.value.code == pbf.ERR_INVALID and "launch_grid" in str(e.value)
 
 
 def test_scenario_configs_shapes():
