@@ -1071,10 +1071,6 @@ int pbf_step_noalloc(const pbf_params *params, float *pos4, float *vel4,
     if (slots > (1LL << 30)) invalid("problem too large (slots beyond 2^30)");
     sc.slots_all = (int)slots;
     pbf::StepShape sh = make_shape(*params, &sc, 1, params, slots, sc.ncells);
-    int dev = 0;
-    PBF_HIP(hipGetDevice(&dev));
-    PBF_HIP(hipDeviceGetAttribute(
-        &sh.cus, hipDeviceAttributeMultiprocessorCount, dev));
     // the buffers of the step, 256-byte aligned, from the start of the
     // scratch: first their extent, then their places
     size_t need_bytes = 0;
@@ -1096,6 +1092,10 @@ int pbf_step_noalloc(const pbf_params *params, float *pos4, float *vel4,
                     n, need_bytes, pbf::kStaticScratchBytes);
       invalid(msg);
     }
+    int dev = 0;
+    PBF_HIP(hipGetDevice(&dev));
+    PBF_HIP(hipDeviceGetAttribute(
+        &sh.cus, hipDeviceAttributeMultiprocessorCount, dev));
     void *base = nullptr;
     PBF_HIP(pbf::static_scratch(&base));
     need_bytes = 0;
