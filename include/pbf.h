@@ -89,9 +89,12 @@ typedef struct pbf_params {
                           /* block_size slots; g > 0 (a multiple of 8) =    */
                           /* persistent, g work-groups walking all tiles;   */
                           /* -1 = persistent, as many work-groups as the    */
-                          /* device holds at once.  block_size 0 and        */
-                          /* launch_grid 0: both tuned                      */
-  int32_t reserved[6];
+                          /* device holds at once.                          */
+  int32_t scan_cells;     /* cells per work-group of the cell-count scan:   */
+                          /* 1024, 2048 or 4096; 0 = 1024 or tuned.         */
+                          /* block_size, launch_grid and scan_cells all 0:  */
+                          /* all three tuned                                */
+  int32_t reserved[5];
 } pbf_params;
 
 /* One input particle, layout of the reference's Particle
@@ -186,10 +189,10 @@ const char *pbf_last_error(void);
  * file ("key value" lines; the reference keys plus the physics keys
  * physics ref|paper, iterations, relaxation, scorr_k, scorr_n, scorr_dq,
  * xsph_c, vorticity_eps, gravity_y, precision fp32|mixed, fp32_iterations,
- * sphere cx cy cz r, block_size, launch_grid).  Missing keys keep the REF
- * defaults of pbf_params_init (zero for the box).  part_input_file is
- * copied into part_file (NUL-terminated, at most cap bytes) when
- * part_file != NULL. */
+ * sphere cx cy cz r, block_size, launch_grid, scan_cells).  Missing keys
+ * keep the REF defaults of pbf_params_init (zero for the box).
+ * part_input_file is copied into part_file (NUL-terminated, at most cap
+ * bytes) when part_file != NULL. */
 int pbf_read_par(const char *path, pbf_params *out, char *part_file,
                  size_t cap);
 
@@ -298,11 +301,14 @@ int pbf_get_config(const pbf_sim *sim, int32_t cfg[4]);
  * work-group per tile of block[c] slots, the work-group count of a
  * persistent launch, or -1 for a persistent launch of as many work-groups
  * as the device holds at once (the autotuner times the per-tile and the
- * resident persistent form; launch_grid forces one).  Fills min(n, 9) entries; *count (if not NULL) receives 9.  No
- * reference counterpart (the reference launches every kernel over the
- * particle count rounded up to 32, Simulation.cpp:138-141). */
+ * resident persistent form; launch_grid forces one).  Fills min(n, 9)
+ * entries; *count (if not NULL) receives 9 and *scan_cells (if not NULL)
+ * the cells per work-group of the cell-count scan (1024, 2048 or 4096:
+ * scan_cells, or tuned with the rest).  No reference counterpart (the
+ * reference launches every kernel over the particle count rounded up to
+ * 32, Simulation.cpp:138-141). */
 int pbf_get_launch(const pbf_sim *sim, int32_t *block, int32_t *grid,
-                   int32_t n, int32_t *count);
+                   int32_t n, int32_t *count, int32_t *scan_cells);
 
 #ifdef __cplusplus
 }

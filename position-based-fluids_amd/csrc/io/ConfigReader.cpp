@@ -112,6 +112,8 @@ ConfigParameters ConfigReader::read(const std::string &filename) const {
         ss >> p.physics.block_size;
       } else if (key == "launch_grid") {
         ss >> p.physics.launch_grid;
+      } else if (key == "scan_cells") {
+        ss >> p.physics.scan_cells;
       } else {
         std::cerr << "Unknown parameter " << key << std::endl
                   << "Leaving it out." << std::endl;

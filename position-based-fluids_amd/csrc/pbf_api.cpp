@@ -116,6 +116,9 @@ void validate(const pbf_params &p) {
       (p.launch_grid < 0 || p.launch_grid % 8 != 0 ||
        p.launch_grid > (1 << 20)))
     invalid("launch_grid must be -1, 0 or a multiple of 8 up to 2^20");
+  if (p.scan_cells != 0 && p.scan_cells != 1024 && p.scan_cells != 2048 &&
+      p.scan_cells != 4096)
+    invalid("scan_cells must be 0, 1024, 2048 or 4096");
 }
 
 pbf::Scene make_scene(const pbf_params &p) {
@@ -303,7 +306,9 @@ pbf::StepShape make_shape(const pbf_params &p0, const pbf::Scene *scenes,
   pbf::StepShape sh{};
   sh.slots = (int)slots;
   sh.cells = (int)cells;
-  sh.scan_blocks = (int)((cells + pbf::kScanTile) / pbf::kScanTile);
+  sh.scan_cells = (int)((cells + pbf::kScanTileMax) / pbf::kScanTileMax *
+                         pbf::kScanTileMax);
+  pbf::set_scan_v(sh, p0.scan_cells ? p0.scan_cells / pbf::kScanTile : 1);
   sh.physics = p0.physics == PBF_PHYSICS_REF ? pbf::kRef : pbf::kPaper;
   sh.iterations = p0.iterations;
   // the lambda passes of iterations 1 .. I - F - 1, XSPH and the vorticity
@@ -341,12 +346,10 @@ void layout_buffers(pbf::DevBuffers &d, const pbf::StepShape &sh,
   take(d.pred_u, P);
   take(d.key, P);
   take(d.rank, P);
-  const size_t Mpad = (size_t)sh.scan_blocks * pbf::kScanTile;  // >= M + 1
+  const size_t Mpad = (size_t)sh.scan_cells;  // >= M + 1
   take(d.count, Mpad);
   take(d.cell_start, Mpad);
-  const size_t Tpad = ((size_t)sh.scan_blocks + pbf::kTopChunk) /
-                      pbf::kTopChunk * pbf::kTopChunk;  // whole scan rounds
-  take(d.partial, Tpad);
+  take(d.partial, pbf::scan_partial_entries(sh));
   take(d.perm, P);
   take(d.ppid, P);
   take(d.pid_s, P);
@@ -430,16 +433,16 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   sh = make_shape(p0, S.scenes.data(), S.scenes.size(), params, slot, cells);
   PBF_HIP(hipDeviceGetAttribute(&sh.cus, hipDeviceAttributeMultiprocessorCount,
                                 S.device));
-  S.tune_pending = p0.block_size == 0 && p0.launch_grid == 0;
+  S.tune_pending =
+      p0.block_size == 0 && p0.launch_grid == 0 && p0.scan_cells == 0;
 
   const size_t P = (size_t)sh.slots;
   pbf::DevBuffers &d = S.d;
   layout_buffers(d, sh, nscenes, [&](size_t bytes) {
     return static_cast<void *>(S.alloc<char>(bytes));
   });
-  const size_t Mpad = (size_t)sh.scan_blocks * pbf::kScanTile;
-  const size_t Tpad = ((size_t)sh.scan_blocks + pbf::kTopChunk) /
-                      pbf::kTopChunk * pbf::kTopChunk;
+  const size_t Mpad = (size_t)sh.scan_cells;
+  const size_t Tpad = pbf::scan_partial_entries(sh);
   S.diag_out = S.alloc<double>(8);
   PBF_HIP(hipHostMalloc((void **)&S.scenes_pinned,
                         nscenes * sizeof(pbf::Scene), hipHostMallocDefault));
@@ -480,13 +483,16 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
 // the device holds at once, walking all tiles), by timing each class over two
 // steps in every combination on the handle's own state (saved before and
 // restored after); the classes are independent launches, so the fastest
-// combination of each is taken.  Results are kept per problem shape for the
-// process and, when PBF_TUNE_CACHE names a file, in that file ("key b g b g
-// ..." lines, size and grid per class).  Handles below kTuneMinSlots slots
+// combination of each is taken.  Then the cells per work-group of the
+// cell-count scan: 1024, 2048 or 4096.  Results are kept per problem shape
+// for the process and, when PBF_TUNE_CACHE names a file, in that file
+// ("key b g b g ... v" lines: size and grid per class, then the scan's int4
+// counts per work-item).  Handles below kTuneMinSlots slots
 // keep the default: there the launch overhead, not the kernels, dominates.
 constexpr int kTuneMinSlots = 1 << 18;
 struct Launch {
   std::array<int, pbf::kNumClasses> block, grid;
+  int scan_v = 1;
 };
 
 std::mutex g_tune_mu;
@@ -501,7 +507,7 @@ std::string tune_key(const pbf_sim &S) {
   k << arch << ":" << S.shape.slots << ":" << S.shape.cells << ":"
     << S.shape.physics << ":" << S.shape.iterations << ":"
     << S.shape.fp16_iterations << ":" << S.shape.vorticity << ":"
-    << S.scenes.size() << ":c" << pbf::kNumClasses << ":t2";
+    << S.scenes.size() << ":c" << pbf::kNumClasses << ":t3";
   return k.str();
 }
 
@@ -529,6 +535,8 @@ bool tune_lookup(const std::string &key, Launch &out) {
       for (int c = 0; c < pbf::kNumClasses; ++c)
         ok = ok && (ls >> l.block[c] >> l.grid[c]) && valid_block(l.block[c]) &&
              valid_grid(l.grid[c]);
+      ok = ok && (ls >> l.scan_v) &&
+           (l.scan_v == 1 || l.scan_v == 2 || l.scan_v == 4);
       if (ok) {
         g_tune[key] = l;
         out = l;
@@ -547,13 +555,14 @@ void tune_store(const std::string &key, const Launch &l) {
     f << key;
     for (int c = 0; c < pbf::kNumClasses; ++c)
       f << " " << l.block[c] << " " << l.grid[c];
-    f << "\n";
+    f << " " << l.scan_v << "\n";
   }
 }
 
 void apply_launch(pbf::StepShape &sh, const Launch &l) {
   std::copy(l.block.begin(), l.block.end(), sh.block);
   std::copy(l.grid.begin(), l.grid.end(), sh.grid);
+  pbf::set_scan_v(sh, l.scan_v);
 }
 
 void autotune(pbf_sim &S) {
@@ -589,6 +598,27 @@ void autotune(pbf_sim &S) {
     best.grid.fill(0);
     std::array<double, pbf::kNumClasses> best_ms;
     best_ms.fill(1e30);
+    // two timed steps of the current shape from the saved state: ms per
+    // kernel class
+    auto trial = [&] {
+      copy(S.d.pos, spos, P * sizeof(float4));
+      copy(S.d.vel, svel, P * sizeof(float4));
+      copy(S.d.pid, spid, P * sizeof(int));
+      pbf::enqueue_predict(S.d, S.shape, S.stream);
+      pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);  // warm-up
+      EventTimer timer;
+      for (int t = 0; t < 2; ++t)
+        pbf::enqueue_step(S.d, S.shape, S.stream, &timer);
+      PBF_HIP(hipStreamSynchronize(S.stream));
+      std::array<double, pbf::kNumClasses> ms{};
+      for (const auto &r : timer.recs) {
+        float t = 0.0f;
+        PBF_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        ms[r.cls] += t;
+      }
+      return ms;
+    };
+    pbf::set_scan_v(S.shape, 1);
     for (int b : {64, 128, 256}) {
       // one work-group per tile, then persistent with the work-groups the
       // device holds at once
@@ -597,27 +627,24 @@ void autotune(pbf_sim &S) {
           S.shape.block[c] = b;
           S.shape.grid[c] = g;
         }
-        copy(S.d.pos, spos, P * sizeof(float4));
-        copy(S.d.vel, svel, P * sizeof(float4));
-        copy(S.d.pid, spid, P * sizeof(int));
-        pbf::enqueue_predict(S.d, S.shape, S.stream);
-        pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);  // warm-up
-        EventTimer timer;
-        for (int t = 0; t < 2; ++t)
-          pbf::enqueue_step(S.d, S.shape, S.stream, &timer);
-        PBF_HIP(hipStreamSynchronize(S.stream));
-        std::array<double, pbf::kNumClasses> ms{};
-        for (const auto &r : timer.recs) {
-          float t = 0.0f;
-          PBF_HIP(hipEventElapsedTime(&t, r.a, r.b));
-          ms[r.cls] += t;
-        }
+        const auto ms = trial();
         for (int c = 0; c < pbf::kNumClasses; ++c)
           if (ms[c] < best_ms[c]) {
             best_ms[c] = ms[c];
             best.block[c] = b;
             best.grid[c] = g;
           }
+      }
+    }
+    // the scan's tile at the chosen launches (its time at 1024 cells per
+    // work-group is best_ms[kClsScan])
+    apply_launch(S.shape, best);
+    for (int v : {2, 4}) {
+      pbf::set_scan_v(S.shape, v);
+      const auto ms = trial();
+      if (ms[pbf::kClsScan] < best_ms[pbf::kClsScan]) {
+        best_ms[pbf::kClsScan] = ms[pbf::kClsScan];
+        best.scan_v = v;
       }
     }
     copy(S.d.pos, spos, P * sizeof(float4));
@@ -1152,7 +1179,7 @@ int pbf_calibrate_copy(uint64_t bytes, uint32_t reps, double *ms) {
 }
 
 int pbf_get_launch(const pbf_sim *S, int32_t *block, int32_t *grid,
-                   int32_t n, int32_t *count) {
+                   int32_t n, int32_t *count, int32_t *scan_cells) {
   return guard([&] {
     need(S != nullptr, "sim is NULL");
     need(n >= 0 && (n == 0 || (block && grid)), "bad argument");
@@ -1161,6 +1188,7 @@ int pbf_get_launch(const pbf_sim *S, int32_t *block, int32_t *grid,
       grid[c] = S->shape.grid[c];
     }
     if (count) *count = pbf::kNumClasses;
+    if (scan_cells) *scan_cells = pbf::kScanTile * S->shape.scan_v;
   });
 }
 

@@ -569,6 +569,22 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total) {
   return block_excl_scan<4>(v, lds, total);
 }
 
+// A tile of the cell-count scan: 256 work-items x V int4 (4 V consecutive
+// cells per work-item, 1024 V cells per work-group).
+template <int V>
+__device__ __forceinline__ int tile_counts(const int4 *__restrict__ count,
+                                           int4 (&c)[V]) {
+  const int4 *p = count + ((size_t)blockIdx.x * 256 + threadIdx.x) * V;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    c[k] = p[k];
+    s += c[k].x + c[k].y + c[k].z + c[k].w;
+  }
+  return s;
+}
+
+template <int V>
 __global__ __launch_bounds__(256) void k_scan_reduce(
     const int4 *__restrict__ count, int *__restrict__ partial,
     int *__restrict__ dense) {
@@ -576,9 +592,9 @@ __global__ __launch_bounds__(256) void k_scan_reduce(
   // the dense-cell list of this step's reorder starts empty (its last
   // reader, the previous step's k_reorder_dense, has finished)
   if (blockIdx.x == 0 && threadIdx.x < 2) dense[threadIdx.x] = 0;
-  const int4 c = count[blockIdx.x * 256 + threadIdx.x];
+  int4 c[V];
   int tot;
-  block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot);
+  block_excl_scan256(tile_counts<V>(count, c), lds, tot);
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
@@ -634,27 +650,35 @@ __global__ __launch_bounds__(kTopThreads) void k_scan_top(
 
 // cell_start = exclusive scan of the counts, tile by tile from the tile's
 // offset (k_scatter zeroes the counts of the occupied cells afterwards).
+template <int V>
 __device__ __forceinline__ void scan_down_tile(
     const int4 *__restrict__ count, int4 *__restrict__ cell_start, int offset,
     int *lds) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  const int4 c = count[t];
+  int4 c[V];
   int tot;
-  const int run = block_excl_scan256(c.x + c.y + c.z + c.w, lds, tot) + offset;
-  cell_start[t] = make_int4(run, run + c.x, run + c.x + c.y,
-                            run + c.x + c.y + c.z);
+  int run =
+      block_excl_scan256(tile_counts<V>(count, c), lds, tot) + offset;
+  int4 *q = cell_start + ((size_t)blockIdx.x * 256 + threadIdx.x) * V;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    q[k] = make_int4(run, run + c[k].x, run + c[k].x + c[k].y,
+                     run + c[k].x + c[k].y + c[k].z);
+    run += c[k].x + c[k].y + c[k].z + c[k].w;
+  }
 }
 
 // After k_scan_top: partial holds the exclusive scan of the tile totals.
+template <int V>
 __global__ __launch_bounds__(256) void k_scan_down(
     const int4 *__restrict__ count, const int *__restrict__ partial,
     int4 *__restrict__ cell_start) {
   __shared__ int lds[4];
-  scan_down_tile(count, cell_start, partial[blockIdx.x], lds);
+  scan_down_tile<V>(count, cell_start, partial[blockIdx.x], lds);
 }
 
 // Straight after k_scan_reduce (up to kScanSumTiles tiles): partial holds
 // the tile totals, and each tile sums those before it.
+template <int V>
 __global__ __launch_bounds__(256) void k_scan_down_sum(
     const int4 *__restrict__ count, const int *__restrict__ partial,
     int4 *__restrict__ cell_start) {
@@ -663,7 +687,7 @@ __global__ __launch_bounds__(256) void k_scan_down_sum(
   for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) before += partial[k];
   int offset;
   block_excl_scan256(before, lds, offset);
-  scan_down_tile(count, cell_start, offset, lds);
+  scan_down_tile<V>(count, cell_start, offset, lds);
 }
 
 // Also zeroes the count of every occupied cell for the next step's count
@@ -1829,6 +1853,25 @@ int resident_grid(const void *k, int block, size_t smem, int cus) {
 #define PBF_GATHER(L, cls, kp, k1, lds_ints, ...) \
   PBF_TILES(L, cls, (L).sh.block[cls], kp, k1, lds_ints, __VA_ARGS__)
 
+// cell_start from the cell counts, V int4 counts per work-item.
+template <int V>
+static void scan_stage(const DevBuffers &d, Launcher &L) {
+  const StepShape &sh = L.sh;
+  const int4 *count = reinterpret_cast<const int4 *>(d.count);
+  int4 *cs = reinterpret_cast<int4 *>(d.cell_start);
+  PBF_LAUNCH(L, kClsScan, k_scan_reduce<V>, sh.scan_blocks, 256, count,
+             d.partial, d.dense);
+  if (sh.scan_blocks <= kScanSumTiles) {
+    PBF_LAUNCH(L, kClsScan, k_scan_down_sum<V>, sh.scan_blocks, 256, count,
+               d.partial, cs);
+  } else {
+    PBF_LAUNCH(L, kClsScan, k_scan_top, 1, kTopThreads, d.partial,
+               sh.scan_blocks);
+    PBF_LAUNCH(L, kClsScan, k_scan_down<V>, sh.scan_blocks, 256, count,
+               d.partial, cs);
+  }
+}
+
 // The sort of a step.  with_predict: predict, bin and count here (probes,
 // diagnostics); otherwise the previous step's last kernel (or
 // enqueue_predict) has done it and the counts are in place.
@@ -1842,19 +1885,12 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
     PBF_LAUNCH(L, kClsPredict, k_predict_count<PH>, grid, bs, d.scenes,
                d.block_scene, d.pos, d.vel, d.pred_u, d.key, d.rank, d.count,
                predict);
-  PBF_LAUNCH(L, kClsScan, k_scan_reduce, sh.scan_blocks, 256,
-             reinterpret_cast<const int4 *>(d.count), d.partial, d.dense);
-  if (sh.scan_blocks <= kScanSumTiles) {
-    PBF_LAUNCH(L, kClsScan, k_scan_down_sum, sh.scan_blocks, 256,
-               reinterpret_cast<const int4 *>(d.count), d.partial,
-               reinterpret_cast<int4 *>(d.cell_start));
-  } else {
-    PBF_LAUNCH(L, kClsScan, k_scan_top, 1, kTopThreads, d.partial,
-               sh.scan_blocks);
-    PBF_LAUNCH(L, kClsScan, k_scan_down, sh.scan_blocks, 256,
-               reinterpret_cast<const int4 *>(d.count), d.partial,
-               reinterpret_cast<int4 *>(d.cell_start));
-  }
+  if (sh.scan_v == 4)
+    scan_stage<4>(d, L);
+  else if (sh.scan_v == 2)
+    scan_stage<2>(d, L);
+  else
+    scan_stage<1>(d, L);
   PBF_TILES(L, kClsScatter, bs, (k_scatter<true>), (k_scatter<false>), 0,
             d.scenes, d.block_scene, d.key, d.rank, d.pid, d.cell_start,
             d.count, d.perm, d.ppid, d.err);
@@ -2051,7 +2087,7 @@ bool enqueue_repeat(const DevBuffers &d, const StepShape &sh, hipStream_t st,
 static void clear_counts(const DevBuffers &d, const StepShape &sh,
                          hipStream_t st) {
   (void)hipMemsetAsync(d.count, 0,
-                       (size_t)sh.scan_blocks * kScanTile * sizeof(int), st);
+                       (size_t)sh.scan_cells * sizeof(int), st);
 }
 
 void enqueue_predict(const DevBuffers &d, const StepShape &sh,
@@ -2091,9 +2127,8 @@ void enqueue_static_setup(const DevBuffers &d, const StepShape &sh,
     k_fill_int<<<std::max(grid, 1), 256, 0, st>>>(a, v, n);
   };
   fill(d.block_scene, 0, (size_t)sh.slots / kSlotAlign);
-  fill(d.count, 0, (size_t)sh.scan_blocks * kScanTile);
-  fill(d.partial, 0,
-       ((size_t)sh.scan_blocks + kTopChunk) / kTopChunk * kTopChunk);
+  fill(d.count, 0, (size_t)sh.scan_cells);
+  fill(d.partial, 0, scan_partial_entries(sh));
   fill(d.err, 0, 1);
   fill(d.dense, 0, 2);
 }

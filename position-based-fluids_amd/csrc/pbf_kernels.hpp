@@ -9,12 +9,15 @@ namespace pbf {
 constexpr int kRef = 0;
 constexpr int kPaper = 1;
 
-// Cell-count scan: 256 work-items x 4 counts (one int4) per work-group.
-// The count and cell_start arrays are allocated to a whole number of tiles
-// of at least M + 1 entries (the padding counts stay 0), so the scan needs
-// no bounds tests and writes cell_start[M] = total by itself.
-constexpr int kScanIpt = 4;
-constexpr int kScanTile = 256 * kScanIpt;
+// Cell-count scan: 256 work-items x V int4 counts per work-group (a tile of
+// kScanTile V cells, V = 1, 2 or 4: the cells per work-group the
+// autotuner picks).  The count and cell_start arrays are allocated to a
+// whole number of the largest tiles, at least M + 1 entries (the padding
+// counts stay 0), so the scan needs no bounds tests at any V and writes
+// cell_start[M] = total by itself.
+constexpr int kScanTile = 1024;
+constexpr int kScanMaxV = 4;
+constexpr int kScanTileMax = kScanTile * kScanMaxV;
 // Up to this many tiles, each tile of the down-sweep sums the tile totals
 // before it itself (two launches); beyond, a one-work-group scan of the
 // tile totals runs in between (three launches), since the summing work
@@ -60,7 +63,10 @@ enum KernelClass {
 struct StepShape {
   int slots;        // padded particle slots P (multiple of kSlotAlign)
   int cells;        // M, total cells over scenes
-  int scan_blocks;  // ceil((M + 1) / kScanTile)
+  int scan_cells;   // count / cell_start entries: M + 1 padded to a
+                    // multiple of kScanTileMax
+  int scan_v;       // int4 counts per work-item of the scan (1, 2 or 4)
+  int scan_blocks;  // scan tiles: scan_cells / (kScanTile scan_v)
   int physics;      // kRef / kPaper (uniform over scenes)
   int iterations;
   int fp16_iterations;  // leading iterations gathering FP16 positions
@@ -74,6 +80,19 @@ struct StepShape {
   int grid[kNumClasses];
   int cus;          // compute units of the device
 };
+
+// Scan tiles of scan_v int4 per work-item.
+inline void set_scan_v(StepShape &sh, int v) {
+  sh.scan_v = v;
+  sh.scan_blocks = sh.scan_cells / (kScanTile * v);
+}
+
+// Entries of the scan's tile-total array: the tile count at scan_v 1,
+// rounded up to whole rounds of the one-work-group scan.
+inline size_t scan_partial_entries(const StepShape &sh) {
+  return ((size_t)sh.scan_cells / kScanTile + kTopChunk) / kTopChunk *
+         kTopChunk;
+}
 
 // Per-kernel-class event timing (pbf_profile_steps, the autotuner): hands
 // out the start/stop event pair of the next launch of class cls.

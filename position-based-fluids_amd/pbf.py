@@ -67,7 +67,8 @@ class Params(ctypes.Structure):
         ("sphere", ctypes.c_float * 4),
         ("block_size", ctypes.c_int32),
         ("launch_grid", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("scan_cells", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
     def copy(self) -> "Params":
@@ -139,7 +140,7 @@ _SIGNATURES = [
     ("pbf_get_config", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32)]),
     ("pbf_get_launch", ctypes.c_int,
      [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32), ctypes.c_int32,
-      _P(ctypes.c_int32)]),
+      _P(ctypes.c_int32), _P(ctypes.c_int32)]),
     ("pbf_calibrate_copy", ctypes.c_int,
      [ctypes.c_uint64, ctypes.c_uint32, _P(ctypes.c_double)]),
     ("pbf_step_noalloc", ctypes.c_int,
@@ -332,15 +333,19 @@ class Simulation:
 
     def launch(self) -> dict:
         """Launch form per kernel class: {class: (work-group size, grid)},
-        grid 0 for one work-group per tile, else the work-group count of a
-        persistent launch."""
+        grid 0 for one work-group per tile, -1 for a persistent launch of
+        the resident work-groups, else the work-group count of a persistent
+        launch; and "scan_cells": cells per work-group of the cell-count
+        scan."""
         block = (ctypes.c_int32 * 16)()
         grid = (ctypes.c_int32 * 16)()
         count = ctypes.c_int32()
+        cells = ctypes.c_int32()
         _check(self._lib.pbf_get_launch(self._h, block, grid, 16,
-                                        ctypes.byref(count)))
-        return {KERNEL_CLASSES[c]: (block[c], grid[c])
-                for c in range(count.value)}
+                                        ctypes.byref(count), ctypes.byref(cells)))
+        out = {KERNEL_CLASSES[c]: (block[c], grid[c]) for c in range(count.value)}
+        out["scan_cells"] = cells.value
+        return out
 
     def close(self) -> None:
         if getattr(self, "_h", None):

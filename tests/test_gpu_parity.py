@@ -823,14 +823,18 @@ def test_autotuner_restores_state_and_writes_cache(tmp_path, monkeypatch):
     fields = lines[0].split()
     assert fields[0].startswith("gfx")
     vals = [int(f) for f in fields[1:]]
-    assert len(vals) == 2 * len(pbf.KERNEL_CLASSES)
-    assert all(b in (64, 128, 256) for b in vals[0::2])
-    assert all(g in (0, -1) for g in vals[1::2])   # per tile or resident
+    nc = len(pbf.KERNEL_CLASSES)
+    assert len(vals) == 2 * nc + 1
+    assert all(b in (64, 128, 256) for b in vals[0:2 * nc:2])
+    assert all(g in (0, -1) for g in vals[1:2 * nc:2])   # per tile or resident
+    assert vals[-1] in (1, 2, 4)   # int4 counts per work-item of the scan
     # a new handle of the same shape takes the cached choice
     with pbf.Simulation(sc.params[0], sc.particles[0]) as s:
         s.step(1)
         got = s.launch()
-    assert [got[c] for c in pbf.KERNEL_CLASSES] == list(zip(vals[0::2], vals[1::2]))
+    assert [got[c] for c in pbf.KERNEL_CLASSES] == \
+        list(zip(vals[0:2 * nc:2], vals[1:2 * nc:2]))
+    assert got["scan_cells"] == 1024 * vals[-1]
 
 
 @pytest.mark.parametrize("physics", [pbf.PHYSICS_REF, pbf.PHYSICS_PAPER])
@@ -1083,20 +1087,29 @@ def test_full_size_c2_properties():
     assert np.array_equal(pos, pos2) and np.array_equal(vel, vel2)
 
 
-@pytest.mark.parametrize("box_x, tiles", [(16.0, 4001), (16.8, 4201)])
-def test_cell_table_on_both_scan_paths(box_x, tiles):
-    """The cell table is the exclusive scan of the per-cell counts on both
-    sides of the two-launch scan's limit of 4096 tiles of 1024 cells: a
-    16 x 8 x 8 box has 640 x 80 x 80 fine cells (4001 tiles), a 16.8 x 8 x 8
-    box 672 x 80 x 80 (4201 tiles, three launches)."""
-    sc = scenarios.small_dam(dims=(16, 16, 16), box=(box_x, 8.0, 8.0),
+@pytest.mark.parametrize("scan_cells, box, two_launch", [
+    (1024, (16.0, 8.0, 8.0), True), (1024, (16.8, 8.0, 8.0), False),
+    (2048, (16.0, 8.0, 16.0), True), (2048, (16.8, 8.0, 16.0), False),
+    (4096, (16.0, 16.0, 16.0), True), (4096, (16.8, 16.0, 16.0), False)])
+def test_cell_table_on_both_scan_paths(scan_cells, box, two_launch):
+    """The cell table is the exclusive scan of the per-cell counts at each
+    tile size of the scan (1024, 2048, 4096 cells per work-group) on both
+    sides of the two-launch scan's limit of 4096 tiles: e.g. at 1024 cells
+    a 16 x 8 x 8 box has 640 x 80 x 80 fine cells (4004 tiles with the
+    padding to whole 4096-cell tiles), a 16.8 x 8 x 8 box 672 x 80 x 80
+    (4204 tiles, three launches)."""
+    sc = scenarios.small_dam(dims=(16, 16, 16), box=box,
                              physics=pbf.PHYSICS_PAPER, seed=3)
-    say("scan paths", box_x, tiles)
-    with pbf.Simulation(sc.params[0], sc.particles[0]) as s:
+    p = sc.params[0].copy()
+    p.scan_cells = scan_cells
+    say("scan paths", scan_cells, box)
+    with pbf.Simulation(p, sc.particles[0]) as s:
+        assert s.launch()["scan_cells"] == scan_cells
         s.step(2)
         m = s.num_cells()
         g = s.probe()
-    assert (m + 1 + 1023) // 1024 == tiles
+    tiles = (m + 1 + 4095) // 4096 * 4096 // scan_cells
+    assert (tiles <= 4096) == two_launch
     cs = g["cell_start"]
     counts = np.bincount(g["cell_of"], minlength=m)
     assert cs[0] == 0 and np.array_equal(cs[1:], np.cumsum(counts))

@@ -59,11 +59,11 @@ def test_read_par_physics_keys_and_unknown_key(tmp_path):
     f = tmp_path / "s.par"
     f.write_text("part_input_file a.in\nx_max 2\nx_n 20\nphysics paper\n"
                  "iterations 3\nscorr_k 0.002\nsphere 1 2 3 0.5\nbogus 1\n"
-                 "launch_grid 2048\n")
+                 "launch_grid 2048\nscan_cells 4096\n")
     p, part = pbf.read_par(str(f))
     assert part == "a.in" and p.physics == pbf.PHYSICS_PAPER
     assert p.iterations == 3 and p.scorr_k == np.float32(0.002)
-    assert p.launch_grid == 2048
+    assert p.launch_grid == 2048 and p.scan_cells == 4096
     assert list(p.sphere) == [1.0, 2.0, 3.0, 0.5]
     assert p.x_max == 2.0 and p.x_n == 20.0
 
@@ -186,6 +186,12 @@ def test_create_rejects_bad_params():
         with pytest.raises(pbf.PBFError) as e:
             pbf.Simulation(bad, parts)
         assert e.value.code == pbf.ERR_INVALID and "launch_grid" in str(e.value)
+    for cells in (-1, 512, 3072, 8192):
+        bad = p.copy()
+        bad.scan_cells = cells
+        with pytest.raises(pbf.PBFError) as e:
+            pbf.Simulation(bad, parts)
+        assert e.value.code == pbf.ERR_INVALID and "scan_cells" in str(e.value)
 
 
 def test_scenario_configs_shapes():
