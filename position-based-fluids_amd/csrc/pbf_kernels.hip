@@ -590,7 +590,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(
     int *__restrict__ dense) {
   __shared__ int lds[4];
   // the dense-cell list of this step's reorder starts empty (its last
-  // reader, the previous step's k_reorder_dense, has finished)
+  // reader, the previous step's k_reorder, has finished)
   if (blockIdx.x == 0 && threadIdx.x < 2) dense[threadIdx.x] = 0;
   int4 c[V];
   int tot;
@@ -691,14 +691,17 @@ __global__ __launch_bounds__(256) void k_scan_down_sum(
 }
 
 // Also zeroes the count of every occupied cell for the next step's count
-// (the scan has read them; cells without a particle hold 0 already).
+// (the scan has read them; cells without a particle hold 0 already), and
+// lists each cell of more than kDenseCell slots in dense (count, cursor,
+// then (first slot, end) pairs; the slot of arrival rank 0 lists it) for
+// the bucketed rank of k_reorder.
 template <bool PT>
 __global__ __launch_bounds__(256) void k_scatter(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const int *__restrict__ key, const int *__restrict__ rank,
     const int *__restrict__ pid, const int *__restrict__ cell_start,
     int *__restrict__ count, int *__restrict__ perm, int *__restrict__ ppid,
-    int *__restrict__ err) {
+    int *__restrict__ dense, int *__restrict__ err) {
   for (Tiles<PT> T(scenes); T.more(); T.next()) {
     const int lb = T.tile();
     const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -706,7 +709,8 @@ __global__ __launch_bounds__(256) void k_scatter(
     if (i - sc.slot_off >= sc.n) continue;
     const int k = key[i];
     count[k] = 0;
-    const int slot = cell_start[k] + rank[i] + sc.shift;
+    const int r = rank[i];
+    const int slot = cell_start[k] + r + sc.shift;
     // counts that disagree with the keys would place a slot outside its
     // scene: reported (error bit 2), never written
     if ((unsigned)(slot - sc.slot_off) >= (unsigned)sc.n) {
@@ -715,13 +719,110 @@ __global__ __launch_bounds__(256) void k_scatter(
     }
     perm[slot] = i;
     ppid[slot] = pid[i];
+    if (r == 0) {
+      const int hi =
+          min(cell_start[k + 1] + sc.shift, sc.slot_off + sc.n);
+      if (hi - slot > kDenseCell) {
+        const int q = atomicAdd(&dense[0], 1);
+        reinterpret_cast<int2 *>(dense + 2)[q] = make_int2(slot, hi);
+      }
+    }
+  }
+}
+
+// The rank of each slot of a dense cell among the cell by particle id, in
+// O(m) for ids spread over their range instead of the O(m^2) count of
+// k_reorder's tiles: the cell's ids are counted into kDenseBuckets buckets
+// of equal width over [min id, max id], the counts scanned to bucket
+// starts, the slots placed bucket by bucket into scratch (order within a
+// bucket arbitrary), and a slot's rank is its bucket's start plus the
+// number of smaller ids in its bucket.  The work-groups of k_reorder take
+// the cells k_scatter listed from a cursor once their tiles are done, until
+// the list is exhausted; writes what the tiles write.  Every work-item of
+// the work-group calls it (work-group barriers).
+struct DenseLds {
+  int cnt[kDenseBuckets], start[kDenseBuckets], cur[kDenseBuckets];
+  int item, min_id, max_id;
+};
+
+__device__ __forceinline__ void rank_dense_cells(
+    const int *__restrict__ perm, const int *__restrict__ ppid,
+    const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
+    int *__restrict__ pid_s, float4 *__restrict__ xa,
+    float4 *__restrict__ xold, int *__restrict__ dense,
+    int2 *__restrict__ scratch, DenseLds &L) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int ncell = __builtin_amdgcn_readfirstlane(dense[0]);
+  if (ncell == 0) return;  // the usual case: no dense cell this step
+  const int2 *cells = reinterpret_cast<const int2 *>(dense + 2);
+  for (;;) {
+    // the cursor only grows: once it has passed the list, a work-group
+    // leaves without an atomic (the late ones of a large launch)
+    if (tid == 0)
+      L.item = __hip_atomic_load(&dense[1], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) >= ncell
+                   ? ncell
+                   : atomicAdd(&dense[1], 1);
+    __syncthreads();
+    const int item = L.item;
+    if (item >= ncell) return;  // uniform: every work-group reaches it
+    for (int b = tid; b < kDenseBuckets; b += nt) L.cnt[b] = L.cur[b] = 0;
+    if (tid == 0) {
+      L.min_id = INT_MAX;
+      L.max_id = INT_MIN;
+    }
+    __syncthreads();
+    const int lo = cells[item].x, hi = cells[item].y;
+    int mn = INT_MAX, mx = INT_MIN;
+    for (int t = lo + tid; t < hi; t += nt) {
+      mn = min(mn, ppid[t]);
+      mx = max(mx, ppid[t]);
+    }
+    atomicMin(&L.min_id, mn);
+    atomicMax(&L.max_id, mx);
+    __syncthreads();
+    const int pmin = L.min_id;
+    const int width = (L.max_id - pmin) / kDenseBuckets + 1;
+    for (int t = lo + tid; t < hi; t += nt)
+      atomicAdd(&L.cnt[(ppid[t] - pmin) / width], 1);
+    __syncthreads();
+    if (tid == 0) {
+      int run = 0;
+      for (int b = 0; b < kDenseBuckets; ++b) {
+        L.start[b] = run;
+        run += L.cnt[b];
+      }
+    }
+    __syncthreads();
+    for (int t = lo + tid; t < hi; t += nt) {
+      const int p = ppid[t];
+      const int b = (p - pmin) / width;
+      scratch[lo + L.start[b] + atomicAdd(&L.cur[b], 1)] = make_int2(p, t);
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int q = lo + tid; q < hi; q += nt) {
+      const int2 e = scratch[q];
+      const int b = (e.x - pmin) / width;
+      const int b0 = lo + L.start[b], b1 = b0 + L.cnt[b];
+      int r = L.start[b];
+      for (int u = b0; u < b1; ++u) r += scratch[u].x < e.x ? 1 : 0;
+      const int f = lo + r;
+      const int src = perm[e.y];
+      const float4 pu = pred_u[src];
+      const float4 x = pos[src];
+      pid_s[f] = e.x;
+      xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+      xold[f] = make_float4(x.x, x.y, x.z, pu.w);
+    }
+    __syncthreads();
   }
 }
 
 // Arrival order within a cell is not deterministic; rank each slot among
 // its cell by particle id and move it there, gathering x*, x and the id.
-// A cell of more than kDenseCell slots is listed in dense (count, cursor,
-// then (first slot, end) pairs) and left to k_reorder_dense.
+// A cell of more than kDenseCell slots is skipped by the tiles and ranked
+// by rank_dense_cells once a work-group's tiles are done.
 template <bool PT>
 __global__ __launch_bounds__(256) void k_reorder(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
@@ -729,7 +830,9 @@ __global__ __launch_bounds__(256) void k_reorder(
     const int *__restrict__ key, const int *__restrict__ cell_start,
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
     int *__restrict__ pid_s, float4 *__restrict__ xa,
-    float4 *__restrict__ xold, int *__restrict__ dense) {
+    float4 *__restrict__ xold, int *__restrict__ dense,
+    int2 *__restrict__ dsc) {
+  __shared__ DenseLds dl;
   for (Tiles<PT> T(scenes); T.more(); T.next()) {
     const int lb = T.tile();
     const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -742,14 +845,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     // k_scatter reports, reach past it)
     const int lo = max(cell_start[c] + sc.shift, sc.slot_off);
     const int hi = min(cell_start[c + 1] + sc.shift, sc.slot_off + sc.n);
-    if (hi - lo > kDenseCell) {
-      // ranked by k_reorder_dense; the cell's first slot lists it
-      if (t == lo) {
-        const int k = atomicAdd(&dense[0], 1);
-        reinterpret_cast<int2 *>(dense + 2)[k] = make_int2(lo, hi);
-      }
-      continue;
-    }
+    if (hi - lo > kDenseCell) continue;  // ranked by rank_dense_cells
     int r = 0;
     for (int u = lo; u < hi; ++u) r += (ppid[u] < p) ? 1 : 0;
     const int f = lo + r;
@@ -759,84 +855,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
     xold[f] = make_float4(x.x, x.y, x.z, pu.w);
   }
-}
-
-// The rank of each slot of a dense cell among the cell by particle id, in
-// O(m) for ids spread over their range instead of the O(m^2) count of
-// k_reorder: the cell's ids are counted into kDenseBuckets buckets of equal
-// width over [min id, max id], the counts scanned to bucket starts, the
-// slots placed bucket by bucket into scratch (order within a bucket
-// arbitrary), and a slot's rank is its bucket's start plus the number of
-// smaller ids in its bucket.  Work-groups take listed cells from a cursor
-// until the list is exhausted.  Writes what k_reorder writes.
-__global__ __launch_bounds__(1024) void k_reorder_dense(
-    const int *__restrict__ perm, const int *__restrict__ ppid,
-    const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
-    int *__restrict__ pid_s, float4 *__restrict__ xa,
-    float4 *__restrict__ xold, int *__restrict__ dense,
-    int2 *__restrict__ scratch) {
-  __shared__ int cnt[kDenseBuckets], start[kDenseBuckets], cur[kDenseBuckets];
-  __shared__ int s_item, s_min, s_max;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int ncell = dense[0];
-  if (ncell == 0) return;  // the usual case: no dense cell this step
-  const int2 *cells = reinterpret_cast<const int2 *>(dense + 2);
-  for (;;) {
-    if (tid == 0) s_item = atomicAdd(&dense[1], 1);
-    __syncthreads();
-    const int item = s_item;
-    if (item >= ncell) return;  // uniform: every work-group reaches it
-    for (int b = tid; b < kDenseBuckets; b += nt) cnt[b] = cur[b] = 0;
-    if (tid == 0) {
-      s_min = INT_MAX;
-      s_max = INT_MIN;
-    }
-    __syncthreads();
-    const int lo = cells[item].x, hi = cells[item].y;
-    int mn = INT_MAX, mx = INT_MIN;
-    for (int t = lo + tid; t < hi; t += nt) {
-      mn = min(mn, ppid[t]);
-      mx = max(mx, ppid[t]);
-    }
-    atomicMin(&s_min, mn);
-    atomicMax(&s_max, mx);
-    __syncthreads();
-    const int pmin = s_min;
-    const int width = (s_max - pmin) / kDenseBuckets + 1;
-    for (int t = lo + tid; t < hi; t += nt)
-      atomicAdd(&cnt[(ppid[t] - pmin) / width], 1);
-    __syncthreads();
-    if (tid == 0) {
-      int run = 0;
-      for (int b = 0; b < kDenseBuckets; ++b) {
-        start[b] = run;
-        run += cnt[b];
-      }
-    }
-    __syncthreads();
-    for (int t = lo + tid; t < hi; t += nt) {
-      const int p = ppid[t];
-      const int b = (p - pmin) / width;
-      scratch[lo + start[b] + atomicAdd(&cur[b], 1)] = make_int2(p, t);
-    }
-    __threadfence_block();
-    __syncthreads();
-    for (int q = lo + tid; q < hi; q += nt) {
-      const int2 e = scratch[q];
-      const int b = (e.x - pmin) / width;
-      const int b0 = lo + start[b], b1 = b0 + cnt[b];
-      int r = start[b];
-      for (int u = b0; u < b1; ++u) r += scratch[u].x < e.x ? 1 : 0;
-      const int f = lo + r;
-      const int src = perm[e.y];
-      const float4 pu = pred_u[src];
-      const float4 x = pos[src];
-      pid_s[f] = e.x;
-      xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
-      xold[f] = make_float4(x.x, x.y, x.z, pu.w);
-    }
-    __syncthreads();
-  }
+  rank_dense_cells(perm, ppid, pred_u, pos, pid_s, xa, xold, dense, dsc, dl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1893,12 +1912,10 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
     scan_stage<1>(d, L);
   PBF_TILES(L, kClsScatter, bs, (k_scatter<true>), (k_scatter<false>), 0,
             d.scenes, d.block_scene, d.key, d.rank, d.pid, d.cell_start,
-            d.count, d.perm, d.ppid, d.err);
+            d.count, d.perm, d.ppid, d.dense, d.err);
   PBF_TILES(L, kClsReorder, bs, (k_reorder<true>), (k_reorder<false>), 0,
             d.scenes, d.block_scene, d.perm, d.ppid, d.key, d.cell_start,
-            d.pred_u, d.pos, d.pid_s, d.xa, d.xold, d.dense);
-  PBF_LAUNCH(L, kClsReorder, k_reorder_dense, kDenseBlocks, 1024, d.perm,
-             d.ppid, d.pred_u, d.pos, d.pid_s, d.xa, d.xold, d.dense, d.dsc);
+            d.pred_u, d.pos, d.pid_s, d.xa, d.xold, d.dense, d.dsc);
 }
 
 static NextSort next_sort(const DevBuffers &d) {

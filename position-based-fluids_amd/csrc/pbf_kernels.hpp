@@ -29,12 +29,11 @@ constexpr int kTopThreads = 1024;
 constexpr int kTopIpt = 16;
 constexpr int kTopChunk = kTopThreads * kTopIpt;
 
-// Cells of more than kDenseCell slots are ranked by id in k_reorder_dense
-// (kDenseBlocks work-groups, kDenseBuckets id buckets per cell) instead of
-// k_reorder's count over the cell.
+// Cells of more than kDenseCell slots are ranked by id over kDenseBuckets
+// id buckets (the reorder's work-groups, once their tiles are done) instead
+// of the reorder's count over the cell.
 constexpr int kDenseCell = 256;
-constexpr int kDenseBlocks = 16;
-constexpr int kDenseBuckets = 4096;
+constexpr int kDenseBuckets = 1024;
 
 // pbf_step_noalloc: bytes of the static device scratch (per GPU) that
 // holds every temporary of the call (at 428 bytes per particle slot plus 8

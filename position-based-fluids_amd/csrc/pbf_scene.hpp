@@ -98,9 +98,9 @@ struct DevBuffers {
   int *perm;      // slot -> previous index, arrival order
   int *ppid;      // slot -> particle id, arrival order
   int *pid_s;     // sorted particle id
-  int *dense;     // cells left to k_reorder_dense: count, cursor, then
-                  // (first slot, end) pairs
-  int2 *dsc;      // k_reorder_dense scratch (id, slot), one per slot
+  int *dense;     // cells of more than kDenseCell slots, ranked apart:
+                  // count, cursor, then (first slot, end) pairs
+  int2 *dsc;      // their scratch (id, slot), one per slot
   float4 *xa;     // x* (sorted): bins, rows and neighbour lists of a step
   float4 *xb;     // (x, lambda) of a lambda pass (sorted)
   float4 *xc;     // PAPER: x after a delta pass, .w = rho after the last

@@ -474,9 +474,10 @@ def test_all_particles_in_one_cell():
 
 
 def test_dense_cell_rank_matches_oracle():
-    """1,000 particles in one cell: more than k_reorder ranks by counting
-    (256), so the cell goes to the bucketed rank of k_reorder_dense; the
-    sort and three REF steps are bit-exact against the oracle."""
+    """1,000 particles in one cell: more than k_reorder's tiles rank by
+    counting (256), so the cell goes to the bucketed rank its work-groups
+    run after their tiles; the sort and three REF steps are bit-exact
+    against the oracle."""
     say("edge: dense cell, 1000 particles")
     p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF)
     rng = np.random.default_rng(17)
@@ -497,7 +498,7 @@ def test_dense_cell_rank_matches_oracle():
 
 def test_dense_cell_in_a_batch():
     """A batch whose middle scene puts 1,000 particles in one cell (ranked by
-    k_reorder_dense, whose cell ranges carry the scene's slot shift) between
+    the bucketed rank, whose cell ranges carry the scene's slot shift) between
     two ordinary scenes: every scene bit-identical to its single run, and the
     dense one to the FP32 oracle, over 3 REF steps."""
     say("edge: dense cell in a batch")
@@ -525,9 +526,9 @@ def test_hundred_thousand_particles_in_eight_cells():
     """100,000 particles in a box of 2 x 2 x 2 cells (12,500 per cell): the
     sort's permutation is numpy's stable sort of the ids by cell and the cell
     table the scan of the counts.  k_reorder's count over a cell would be
-    O(m^2) (1.6e8 id comparisons per cell); the bucketed rank of
-    k_reorder_dense is O(m) for spread ids: the reorder class of one step
-    takes about 0.1 ms here (kernel events), asserted below 5 ms."""
+    O(m^2) (1.6e8 id comparisons per cell); the bucketed rank its
+    work-groups run for such cells is O(m) for spread ids: the reorder
+    class of one step is asserted below 5 ms (kernel events)."""
     say("edge: 100000 particles in 8 cells")
     p = scenarios.make_params((1.0, 1.0, 1.0), pbf.PHYSICS_REF)
     p.x_n = p.y_n = p.z_n = 2.0
