@@ -480,14 +480,15 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
 // ---- autotuner ---------------------------------------------------------
 // Chooses the launch of each kernel class: work-group size 64, 128 or 256,
 // each with one work-group per tile or persistent (as many work-groups as
-// the device holds at once, walking all tiles), by timing each class over two
-// steps in every combination on the handle's own state (saved before and
-// restored after); the classes are independent launches, so the fastest
-// combination of each is taken.  Then the cells per work-group of the
-// cell-count scan: 1024, 2048 or 4096.  Results are kept per problem shape
-// for the process and, when PBF_TUNE_CACHE names a file, in that file
-// ("key b g b g ... v" lines: size and grid per class, then the scan's int4
-// counts per work-item).  Handles below kTuneMinSlots slots
+// the device holds at once, walking all tiles), by timing each class over
+// two steps in every combination, from the handle's own state advanced 10
+// steps (the state is saved before and restored after); the classes are
+// independent launches, so the fastest combination of each is taken (the
+// default unless another beats it by more than 2 %).  Then the cells per
+// work-group of the cell-count scan: 1024, 2048 or 4096.  Results are kept
+// per problem shape for the process and, when PBF_TUNE_CACHE names a file,
+// in that file ("key b g b g ... v" lines: size and grid per class, then
+// the scan's int4 counts per work-item).  Handles below kTuneMinSlots slots
 // keep the default: there the launch overhead, not the kernels, dominates.
 constexpr int kTuneMinSlots = 1 << 18;
 struct Launch {
@@ -576,20 +577,34 @@ void autotune(pbf_sim &S) {
     return;
   }
   const size_t P = (size_t)S.shape.slots;
-  float4 *spos = nullptr, *svel = nullptr;
-  int *spid = nullptr;
+  // the handle's state (restored at the end) and the trials' start
+  float4 *opos = nullptr, *ovel = nullptr, *spos = nullptr, *svel = nullptr;
+  int *opid = nullptr, *spid = nullptr;
   auto release = [&] {
-    if (spos) (void)hipFree(spos);
-    if (svel) (void)hipFree(svel);
-    if (spid) (void)hipFree(spid);
+    for (void *p : {(void *)opos, (void *)ovel, (void *)opid, (void *)spos,
+                    (void *)svel, (void *)spid})
+      if (p) (void)hipFree(p);
   };
   auto copy = [&](void *dst, const void *src, size_t bytes) {
     PBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S.stream));
   };
   try {
+    PBF_HIP(hipMalloc(&opos, P * sizeof(float4)));
+    PBF_HIP(hipMalloc(&ovel, P * sizeof(float4)));
+    PBF_HIP(hipMalloc(&opid, P * sizeof(int)));
     PBF_HIP(hipMalloc(&spos, P * sizeof(float4)));
     PBF_HIP(hipMalloc(&svel, P * sizeof(float4)));
     PBF_HIP(hipMalloc(&spid, P * sizeof(int)));
+    copy(opos, S.d.pos, P * sizeof(float4));
+    copy(ovel, S.d.vel, P * sizeof(float4));
+    copy(opid, S.d.pid, P * sizeof(int));
+    // the trials start kTuneLead steps into the run (at the default
+    // launches): a scene's first steps, often a lattice at rest, are not
+    // the ones its run spends its time in
+    constexpr int kTuneLead = 10;
+    pbf::enqueue_predict(S.d, S.shape, S.stream);
+    for (int t = 0; t < kTuneLead; ++t)
+      pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);
     copy(spos, S.d.pos, P * sizeof(float4));
     copy(svel, S.d.vel, P * sizeof(float4));
     copy(spid, S.d.pid, P * sizeof(int));
@@ -618,6 +633,13 @@ void autotune(pbf_sim &S) {
       }
       return ms;
     };
+    // A choice other than the default (128 work-items, one work-group per
+    // tile, 1024-cell scan tiles) is kept only when it is faster by more
+    // than kTuneMargin: the trials time the handle's first steps, and
+    // near-ties there resolve at random (and need not hold later in a run).
+    constexpr double kTuneMargin = 0.02;
+    std::array<double, pbf::kNumClasses> def_ms;
+    def_ms.fill(1e30);
     pbf::set_scan_v(S.shape, 1);
     for (int b : {64, 128, 256}) {
       // one work-group per tile, then persistent with the work-groups the
@@ -628,28 +650,38 @@ void autotune(pbf_sim &S) {
           S.shape.grid[c] = g;
         }
         const auto ms = trial();
-        for (int c = 0; c < pbf::kNumClasses; ++c)
+        for (int c = 0; c < pbf::kNumClasses; ++c) {
+          if (b == 128 && g == 0) def_ms[c] = ms[c];
           if (ms[c] < best_ms[c]) {
             best_ms[c] = ms[c];
             best.block[c] = b;
             best.grid[c] = g;
           }
+        }
       }
     }
-    // the scan's tile at the chosen launches (its time at 1024 cells per
-    // work-group is best_ms[kClsScan])
+    for (int c = 0; c < pbf::kNumClasses; ++c)
+      if (def_ms[c] <= best_ms[c] * (1.0 + kTuneMargin)) {
+        best.block[c] = 128;
+        best.grid[c] = 0;
+      }
+    // the scan's tile at the chosen launches (1024 cells per work-group
+    // timed with the rest)
     apply_launch(S.shape, best);
+    const double scan_def = def_ms[pbf::kClsScan];
+    double scan_best = scan_def;
     for (int v : {2, 4}) {
       pbf::set_scan_v(S.shape, v);
       const auto ms = trial();
-      if (ms[pbf::kClsScan] < best_ms[pbf::kClsScan]) {
-        best_ms[pbf::kClsScan] = ms[pbf::kClsScan];
+      if (ms[pbf::kClsScan] < scan_best) {
+        scan_best = ms[pbf::kClsScan];
         best.scan_v = v;
       }
     }
-    copy(S.d.pos, spos, P * sizeof(float4));
-    copy(S.d.vel, svel, P * sizeof(float4));
-    copy(S.d.pid, spid, P * sizeof(int));
+    if (scan_def <= scan_best * (1.0 + kTuneMargin)) best.scan_v = 1;
+    copy(S.d.pos, opos, P * sizeof(float4));
+    copy(S.d.vel, ovel, P * sizeof(float4));
+    copy(S.d.pid, opid, P * sizeof(int));
     S.predicted = false;  // the trial steps left their own
     S.ensure_predicted();
     PBF_HIP(hipStreamSynchronize(S.stream));
