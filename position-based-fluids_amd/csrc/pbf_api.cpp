@@ -237,7 +237,8 @@ struct pbf_sim {
   int total = 0;  // particles over scenes
   pbf::DevBuffers d{};
   std::vector<void *> allocs;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;        // one step
+  hipGraphExec_t graph_multi = nullptr;  // kGraphSteps steps
   double *diag_out = nullptr;
   float wave = 0.0f;
   bool tune_pending = false;  // block size still to be chosen
@@ -256,6 +257,7 @@ struct pbf_sim {
   ~pbf_sim() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_multi) (void)hipGraphExecDestroy(graph_multi);
     for (void *p : allocs) (void)hipFree(p);
     if (scenes_pinned) (void)hipHostFree(scenes_pinned);
     if (upload_done) (void)hipEventDestroy(upload_done);
@@ -695,18 +697,38 @@ void autotune(pbf_sim &S) {
   release();
 }
 
-void ensure_graph(pbf_sim &S) {
-  autotune(S);
-  if (S.graph) return;
+// Steps per launch of the multi-step graph: a graph launch leaves a gap of
+// about 8 us on the GPU before its first kernel (c1 kernel trace), 1 % of a
+// 1M-particle step, paid once per kGraphSteps steps instead of every step.
+constexpr uint32_t kGraphSteps = 8;
+
+// A graph of `steps` steps of the handle, captured and instantiated.
+hipGraphExec_t capture_steps(pbf_sim &S, uint32_t steps) {
   hipGraph_t g = nullptr;
   PBF_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
-  pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);
+  for (uint32_t t = 0; t < steps; ++t)
+    pbf::enqueue_step(S.d, S.shape, S.stream, nullptr);
   PBF_HIP(hipStreamEndCapture(S.stream, &g));
-  const hipError_t e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+  hipGraphExec_t ge = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess)
     throw ApiError(PBF_ERR_HIP, std::string("hipGraphInstantiate: ") +
                                     hipGetErrorString(e));
+  return ge;
+}
+
+void ensure_graph(pbf_sim &S) {
+  autotune(S);
+  if (!S.graph) S.graph = capture_steps(S, 1);
+  if (!S.graph_multi) S.graph_multi = capture_steps(S, kGraphSteps);
+}
+
+// nsteps steps as graph launches: kGraphSteps at a time, then one at a time.
+void launch_steps(pbf_sim &S, uint32_t nsteps) {
+  for (; nsteps >= kGraphSteps; nsteps -= kGraphSteps)
+    PBF_HIP(hipGraphLaunch(S.graph_multi, S.stream));
+  for (; nsteps > 0; --nsteps) PBF_HIP(hipGraphLaunch(S.graph, S.stream));
 }
 
 void need(bool c, const char *m) {
@@ -792,8 +814,7 @@ int pbf_step(pbf_sim *S, uint32_t nsteps) {
     S->upload_scenes();
     ensure_graph(*S);
     S->ensure_predicted();
-    for (uint32_t t = 0; t < nsteps; ++t)
-      PBF_HIP(hipGraphLaunch(S->graph, S->stream));
+    launch_steps(*S, nsteps);
   });
 }
 
@@ -1054,8 +1075,7 @@ int pbf_time_steps(pbf_sim *S, uint32_t nsteps, double *ms) {
     PBF_HIP(hipEventCreate(&a));
     PBF_HIP(hipEventCreate(&b));
     PBF_HIP(hipEventRecord(a, S->stream));
-    for (uint32_t t = 0; t < nsteps; ++t)
-      PBF_HIP(hipGraphLaunch(S->graph, S->stream));
+    launch_steps(*S, nsteps);
     PBF_HIP(hipEventRecord(b, S->stream));
     PBF_HIP(hipEventSynchronize(b));
     float e = 0.0f;

@@ -1003,6 +1003,29 @@ def test_graph_replays_match_eager_launches():
         b.close()
 
 
+def test_multi_step_graph_matches_single_steps():
+    """pbf_step(n) launches a graph of 8 captured steps n // 8 times and
+    the one-step graph for the rest: 19 steps in one call (two 8-step
+    launches, three single ones), with a moving wall set before the call,
+    equal 19 calls of one step, and eager launches, bit for bit."""
+    sc = scenarios.small_dam(dims=(20, 20, 20), box=(2.0, 2.0, 2.0),
+                             physics=pbf.PHYSICS_PAPER, seed=9)
+    say("multi-step graph", sc.n)
+    with pbf.Simulation(sc.params[0], sc.particles[0]) as a, \
+            pbf.Simulation(sc.params[0], sc.particles[0]) as b, \
+            pbf.Simulation(sc.params[0], sc.particles[0]) as c:
+        for s in (a, b, c):
+            s.set_wave_generator(0.05)
+        a.step(19)
+        for _ in range(19):
+            b.step(1)
+        c.profile(19)
+        pa, va = a.dump()
+        for other in (b, c):
+            po, vo = other.dump()
+            assert np.array_equal(pa, po) and np.array_equal(va, vo)
+
+
 def test_diagnostics_against_fp64_density():
     """pbf_diagnostics on a small block plus one particle far from it: the
     compression maximum runs over all particles, the mean and maximum of
