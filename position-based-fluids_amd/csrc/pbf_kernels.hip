@@ -136,12 +136,15 @@ __device__ __forceinline__ uint4 pack_qv(const Scene &sc, float3 x, float rho,
 // a - b of two packed positions in units of q_unit: each coordinate
 // difference taken modulo 2^16 as a signed 16-bit value (exact for the
 // quantised points while they are less than 2^15 units, 8 h, apart).
+typedef short Short2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ float3 q_diff(unsigned ax, unsigned ay,
                                          unsigned bx, unsigned by) {
-  const int dx = (int)(short)(unsigned short)(ax - bx);
-  const int dy = (int)(short)(unsigned short)((ax >> 16) - (bx >> 16));
+  // x and y in one packed 16-bit subtraction (each half modulo 2^16)
+  const Short2 dxy =
+      __builtin_bit_cast(Short2, ax) - __builtin_bit_cast(Short2, bx);
   const int dz = (int)(short)(unsigned short)(ay - by);
-  return make_float3((float)dx, (float)dy, (float)dz);
+  return make_float3((float)dxy.x, (float)dxy.y, (float)dz);
 }
 
 // x, y, z of a float4 record, read by a 12-byte load at the 16-byte stride
