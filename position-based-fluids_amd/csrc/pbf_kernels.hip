@@ -125,10 +125,18 @@ __device__ __forceinline__ uint2 pack_q(const Scene &sc, float3 x,
   return make_uint2(qx | (qy << 16), qz | (payload << 16));
 }
 
-// XSPH record: position, rho / rho0 and v.
+// 1/rho of the XSPH weights, 0 for a particle with no density (no pair
+// within h in the last lambda pass: it carries no weight).
+__device__ __forceinline__ float inv_density(float rho) {
+  return rho > 0.0f ? __builtin_amdgcn_rcpf(rho) : 0.0f;
+}
+
+// XSPH record: position, rho0 / rho (0 for no density; at most FP16's
+// largest finite value, 65504) and v.
 __device__ __forceinline__ uint4 pack_qv(const Scene &sc, float3 x, float rho,
                                          float3 v) {
-  const uint2 q = pack_q(sc, x, f16_bits(rho * sc.inv_rho0));
+  const float r = fminf(sc.rho0 * inv_density(rho), 65504.0f);
+  const uint2 q = pack_q(sc, x, f16_bits(r));
   return make_uint4(q.x, q.y, f16_bits(v.x) | (f16_bits(v.y) << 16),
                     f16_bits(v.z));
 }
@@ -1453,7 +1461,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
     if constexpr (LAST) {
       const float4 xo = xold[i];
       const float3 v = velocity_exact(sc, xn, xo);
-      XV[i] = XVRec{make_float4(xn.x, xn.y, xn.z, rho[i]),
+      XV[i] = XVRec{make_float4(xn.x, xn.y, xn.z, inv_density(rho[i])),
                     make_float4(v.x, v.y, v.z, xo.w)};
       if constexpr (WRITE_QV) QV[i] = pack_qv(sc, xn, rho[i], v);
     } else {
@@ -1463,11 +1471,11 @@ __global__ __launch_bounds__(256) void k_delta_paper(
 }
 
 // The XSPH sum over N_i: (v_j - v_i) W_ij / rho_j, a neighbour with no
-// density (no pair within h in the last lambda pass) carrying no weight;
-// with OMEGA also omega_i = sum (v_j - v_i) x grad_j W_ij (eq. 15), spiky
-// applied by the caller.  QIN: neighbours from the 16-bit records QV, the
-// sums formed in units of q_unit (scale: s by q_unit^6 / rho0, since the
-// records hold rho / rho0; om by q_unit^2).
+// density (no pair within h in the last lambda pass) carrying no weight
+// (its record holds 1/rho_j = 0); with OMEGA also omega_i = sum (v_j - v_i)
+// x grad_j W_ij (eq. 15), spiky applied by the caller.  QIN: neighbours
+// from the 16-bit records QV, the sums formed in units of q_unit (scale: s
+// by q_unit^6 / rho0, since the records hold rho0 / rho; om by q_unit^2).
 template <bool QIN, bool OMEGA>
 __device__ __forceinline__ void xsph_sums(
     const Scene &sc, const int *__restrict__ cs, const Gather<Pos3> &xsg,
@@ -1476,14 +1484,15 @@ __device__ __forceinline__ void xsph_sums(
     const int *__restrict__ rbase, int stride, int i, int *rb, float3 xi,
     float4 vi, float3 &s, float3 &om) {
   const float h = QIN ? kQH : sc.h, h2 = QIN ? kQH2 : sc.h2;
-  auto term = [&](float rx, float ry, float rz, float rho_j, float3 vj) {
+  // inv_j: 1/rho_j (FP32 records) or rho0/rho_j (16-bit records), 0 for a
+  // neighbour with no density
+  auto term = [&](float rx, float ry, float rz, float inv_j, float3 vj) {
     const float r2 = rx * rx + ry * ry + rz * rz;
     // branch-free: the terms of an out-of-reach entry are selected to 0
     const bool in = in_reach(r2, h2);
     const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
     const float t = h2 - r2;
-    const float w =
-        in && rho_j > 0.0f ? t * t * t * __builtin_amdgcn_rcpf(rho_j) : 0.0f;
+    const float w = in ? t * t * t * inv_j : 0.0f;
     s.x += vx * w;
     s.y += vy * w;
     s.z += vz * w;

@@ -76,7 +76,7 @@ static_assert(sizeof(Scene) % 16 == 0, "Scene must stay 16-byte aligned");
 // Position and velocity of one slot side by side, so that a gather of a
 // neighbour's pair reads one 32-byte record.
 struct XVRec {
-  float4 x;  // x, rho
+  float4 x;  // x, 1/rho (0: no density)
   float4 v;  // v, m
 };
 
@@ -108,14 +108,14 @@ struct DevBuffers {
   float *lam;     // lambda (sorted)
   float *rho;     // density of the last lambda pass (sorted)
   float4 *v;      // REF: velocity after the constraint solve (sorted), .w = m
-  XVRec *xv;      // PAPER: (x, rho), (v, m) after the last delta pass
+  XVRec *xv;      // PAPER: (x, 1/rho), (v, m) after the last delta pass
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
   // mixed precision records (sorted), 16-bit fixed-point x (x | y << 16,
   // z | payload << 16) and FP16 payloads:
   uint2 *qa;      //   x after a delta pass (payload 0), for a lambda pass;
                   //   x and |omega| after XSPH, for the vorticity pass
-  uint4 *qv;      //   x, rho / rho0, v after the last delta pass, for XSPH
+  uint4 *qv;      //   x, rho0 / rho, v after the last delta pass, for XSPH
   uint4 *nl;      // PAPER neighbour lists: kMaxGroups groups of kPerGroup
                   // entries per slot, group g of slot i at [g * P + i]
   int *ncnt;      // list length per slot (> kMaxHits: list overflowed)
