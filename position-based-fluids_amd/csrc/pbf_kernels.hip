@@ -97,6 +97,14 @@ __device__ __forceinline__ bool in_reach(float r2, float h2) {
   return __float_as_uint(r2) - 1u < __float_as_uint(h2) - 1u;
 }
 
+// (h - r)^2 / r, the spiky gradient's magnitude over r, from inv = 1/r:
+// h^2/r - 2 h + r = (h^2 + r^2) inv - 2 h, one add and one fused
+// multiply-add (r = 0 gives inf; callers select it away with in_reach).
+__device__ __forceinline__ float spiky_over_r(float inv, float r2, float h,
+                                              float h2) {
+  return fmaf(inv, h2 + r2, -2.0f * h);
+}
+
 // Mixed precision records.  A position is 16-bit fixed point in units of
 // q_unit = h / 2^kQBits from the scene's lower corner, each coordinate kept
 // modulo 2^16, packed (x | y << 16, z | payload << 16); payloads and the
@@ -1249,7 +1257,8 @@ struct LambdaSums {
     const float t = in ? h2 - r2 : 0.0f;
     const float hr2 = hr * hr;
     wsum += t * t * t;
-    g2 += in ? hr2 * hr2 : 0.0f;
+    const float hm = in ? hr2 : 0.0f;  // hr2 is NaN for r = 0
+    g2 += hm * hm;
     const float gs = in ? hr2 * inv : 0.0f;
     gx += rx * gs;
     gy += ry * gs;
@@ -1436,7 +1445,6 @@ __global__ __launch_bounds__(256) void k_delta_paper(
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
       const float t = h2 - r2;
       const float q = t * t * t * cq;  // W / W(dq)
       float qn;
@@ -1448,7 +1456,8 @@ __global__ __launch_bounds__(256) void k_delta_paper(
         for (int e = 0; e < n; ++e) qn *= q;
       }
       const float coef = li + xj.w - kk * qn;
-      const float gs = in_reach(r2, h2) ? coef * hr * hr * inv : 0.0f;
+      const float gs =
+          in_reach(r2, h2) ? coef * spiky_over_r(inv, r2, h, h2) : 0.0f;
       ax += rx * gs;
       ay += ry * gs;
       az += rz * gs;
@@ -1498,8 +1507,8 @@ __device__ __forceinline__ void xsph_sums(
     s.z += vz * w;
     if constexpr (OMEGA) {
       const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
-      const float gs = in ? hr * hr * inv : 0.0f;  // grad_j W_ij = spiky gs r
+      // grad_j W_ij = spiky gs r
+      const float gs = in ? spiky_over_r(inv, r2, h, h2) : 0.0f;
       const float gx = rx * gs, gy = ry * gs, gz = rz * gs;
       om.x += vy * gz - vz * gy;
       om.y += vz * gx - vx * gz;
@@ -1645,8 +1654,8 @@ __global__ __launch_bounds__(256) void k_vort_final(
     auto term = [&](float rx, float ry, float rz, float dw) {
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float inv = __builtin_amdgcn_rsqf(r2);
-      const float hr = h - r2 * inv;
-      const float gs = in_reach(r2, h2) ? dw * hr * hr * inv : 0.0f;
+      const float gs =
+          in_reach(r2, h2) ? dw * spiky_over_r(inv, r2, h, h2) : 0.0f;
       ex += rx * gs;
       ey += ry * gs;
       ez += rz * gs;
