@@ -97,6 +97,13 @@ __device__ __forceinline__ bool in_reach(float r2, float h2) {
   return __float_as_uint(r2) - 1u < __float_as_uint(h2) - 1u;
 }
 
+// r2 if in reach, else h2: there W = 0 exactly and h - r is within a
+// rounding of 0 (exactly 0 where h2 is a power of 2), so an entry out of
+// reach contributes nothing above the rounding of a sum.
+__device__ __forceinline__ float in_reach_or_h2(float r2, float h2) {
+  return in_reach(r2, h2) ? r2 : h2;
+}
+
 // (h - r)^2 / r, the spiky gradient's magnitude over r, from inv = 1/r:
 // h^2/r - 2 h + r = (h^2 + r^2) inv - 2 h, one add and one fused
 // multiply-add (r = 0 gives inf; callers select it away with in_reach).
@@ -113,8 +120,13 @@ __device__ __forceinline__ unsigned q16(float v, float lo, float scale) {
   return (unsigned)__float2int_rn((v - lo) * scale) & 0xffffu;
 }
 
+// FP16 bits of v, saturated at FP16's largest finite value (+-65504)
+// rather than rounded to infinity: a near-coincident pair can give an
+// |omega| or a velocity beyond FP16's range, which the FP32 passes carry as
+// a large finite value; a NaN stays NaN (the state's own check reports it).
 __device__ __forceinline__ unsigned f16_bits(float v) {
-  return (unsigned)__half_as_ushort(__float2half_rn(v));
+  const float c = v != v ? v : fminf(fmaxf(v, -65504.0f), 65504.0f);
+  return (unsigned)__half_as_ushort(__float2half_rn(c));
 }
 
 __device__ __forceinline__ float f16_lo(unsigned w) {
@@ -143,8 +155,7 @@ __device__ __forceinline__ float inv_density(float rho) {
 // largest finite value, 65504) and v.
 __device__ __forceinline__ uint4 pack_qv(const Scene &sc, float3 x, float rho,
                                          float3 v) {
-  const float r = fminf(sc.rho0 * inv_density(rho), 65504.0f);
-  const uint2 q = pack_q(sc, x, f16_bits(r));
+  const uint2 q = pack_q(sc, x, f16_bits(sc.rho0 * inv_density(rho)));
   return make_uint4(q.x, q.y, f16_bits(v.x) | (f16_bits(v.y) << 16),
                     f16_bits(v.z));
 }
@@ -1247,22 +1258,14 @@ struct LambdaSums {
     gy += ry * gs;
     gz += rz * gs;
   }
-  // add() of a replayed entry with no branch: the terms are formed for every
-  // entry and selected to 0 for one out of reach (r = 0 gives inf and NaN
-  // there, which the selects drop); the sums are those of add().
+  // add() of a replayed entry with no branch: an entry out of reach (r = 0
+  // for the padding, or r >= h after the positions moved) is evaluated at
+  // r^2 = h^2, where t = 0 exactly and h - r is within a rounding of 0
+  // (exactly 0 in units of q_unit, where h^2 = 2^24): its terms vanish to
+  // below the rounding of the sums, with one select instead of three.
   __device__ __forceinline__ void add_if(bool in, float h, float h2, float rx,
                                          float ry, float rz, float r2) {
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float hr = h - r2 * inv;
-    const float t = in ? h2 - r2 : 0.0f;
-    const float hr2 = hr * hr;
-    wsum += t * t * t;
-    const float hm = in ? hr2 : 0.0f;  // hr2 is NaN for r = 0
-    g2 += hm * hm;
-    const float gs = in ? hr2 * inv : 0.0f;
-    gx += rx * gs;
-    gy += ry * gs;
-    gz += rz * gs;
+    add(h, h2, rx, ry, rz, in ? r2 : h2);
   }
   __device__ __forceinline__ void store(const Scene &sc, int i, float3 xi,
                                         float4 *__restrict__ B,
@@ -1496,19 +1499,19 @@ __device__ __forceinline__ void xsph_sums(
   // inv_j: 1/rho_j (FP32 records) or rho0/rho_j (16-bit records), 0 for a
   // neighbour with no density
   auto term = [&](float rx, float ry, float rz, float inv_j, float3 vj) {
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    // branch-free: the terms of an out-of-reach entry are selected to 0
-    const bool in = in_reach(r2, h2);
+    // branch-free: an out-of-reach entry is evaluated at r^2 = h^2, where
+    // W = 0 exactly and the spiky factor is within a rounding of 0
+    const float r2 = in_reach_or_h2(rx * rx + ry * ry + rz * rz, h2);
     const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
     const float t = h2 - r2;
-    const float w = in ? t * t * t * inv_j : 0.0f;
+    const float w = t * t * t * inv_j;
     s.x += vx * w;
     s.y += vy * w;
     s.z += vz * w;
     if constexpr (OMEGA) {
       const float inv = __builtin_amdgcn_rsqf(r2);
       // grad_j W_ij = spiky gs r
-      const float gs = in ? spiky_over_r(inv, r2, h, h2) : 0.0f;
+      const float gs = spiky_over_r(inv, r2, h, h2);
       const float gx = rx * gs, gy = ry * gs, gz = rz * gs;
       om.x += vy * gz - vz * gy;
       om.y += vz * gx - vx * gz;
