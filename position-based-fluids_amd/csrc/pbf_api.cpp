@@ -360,6 +360,7 @@ void layout_buffers(pbf::DevBuffers &d, const pbf::StepShape &sh,
   take(d.xa, P);
   take(d.xb, P);
   if (sh.physics == pbf::kPaper) {
+    take(d.xp, 3 * P + 16);
     take(d.xc, P);
     take(d.xv, P);
   }

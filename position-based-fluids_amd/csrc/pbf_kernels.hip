@@ -218,6 +218,50 @@ struct Gather {
   __device__ __forceinline__ T operator[](int j) const {
     return at_bytes(j * kStride);
   }
+  // records j .. j + 3, all their loads issued before any is used
+  __device__ __forceinline__ void batch4(int j, T &a0, T &a1, T &a2,
+                                         T &a3) const {
+    a0 = (*this)[j];
+    a1 = (*this)[j + 1];
+    a2 = (*this)[j + 2];
+    a3 = (*this)[j + 3];
+  }
+};
+
+// x* of the sorted slots as 3 floats each (12-byte records), for the
+// search's candidates: four consecutive candidates are 48 bytes, read by
+// three 16-byte loads (4-byte aligned), which touch fewer cache lines per
+// wave instruction than four loads at the 16-byte stride of x*.  Loads past
+// the n records return 0.
+struct PackedGather {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ PackedGather(const float *p, int n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0,
+                                            n * 12, 0x00020000)) {}
+  __device__ __forceinline__ Pos3 operator[](int j) const {
+    const U3 v = __builtin_amdgcn_raw_buffer_load_b96(r, j * 12, 0, 0);
+    return Pos3{__uint_as_float(v.x), __uint_as_float(v.y),
+                __uint_as_float(v.z)};
+  }
+  __device__ __forceinline__ void batch4(int j, Pos3 &a0, Pos3 &a1,
+                                         Pos3 &a2, Pos3 &a3) const {
+    const int o = j * 12;
+    const uint4 u = __builtin_bit_cast(
+        uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
+    const uint4 v = __builtin_bit_cast(
+        uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 16, 0, 0));
+    const uint4 w = __builtin_bit_cast(
+        uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 32, 0, 0));
+    a0 = Pos3{__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z)};
+    a1 = Pos3{__uint_as_float(u.w), __uint_as_float(v.x), __uint_as_float(v.y)};
+    a2 = Pos3{__uint_as_float(v.z), __uint_as_float(v.w), __uint_as_float(w.x)};
+    a3 = Pos3{__uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w)};
+  }
+};
+
+// A 12-byte record at 4-byte alignment (x* packed for the search).
+struct __attribute__((packed, aligned(4))) Float3P {
+  float x, y, z;
 };
 
 // Cell coordinates of a position.  REF: (int)((x - SYSTEM_MIN) /
@@ -778,7 +822,7 @@ struct DenseLds {
 __device__ __forceinline__ void rank_dense_cells(
     const int *__restrict__ perm, const int *__restrict__ ppid,
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
-    int *__restrict__ pid_s, float4 *__restrict__ xa,
+    int *__restrict__ pid_s, float4 *__restrict__ xa, float *__restrict__ xp,
     float4 *__restrict__ xold, int *__restrict__ dense,
     int2 *__restrict__ scratch, DenseLds &L) {
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -843,6 +887,7 @@ __device__ __forceinline__ void rank_dense_cells(
       const float4 x = pos[src];
       pid_s[f] = e.x;
       xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+      if (xp) reinterpret_cast<Float3P *>(xp)[f] = Float3P{pu.x, pu.y, pu.z};
       xold[f] = make_float4(x.x, x.y, x.z, pu.w);
     }
     __syncthreads();
@@ -859,7 +904,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     const int *__restrict__ perm, const int *__restrict__ ppid,
     const int *__restrict__ key, const int *__restrict__ cell_start,
     const float4 *__restrict__ pred_u, const float4 *__restrict__ pos,
-    int *__restrict__ pid_s, float4 *__restrict__ xa,
+    int *__restrict__ pid_s, float4 *__restrict__ xa, float *__restrict__ xp,
     float4 *__restrict__ xold, int *__restrict__ dense,
     int2 *__restrict__ dsc) {
   __shared__ DenseLds dl;
@@ -883,9 +928,11 @@ __global__ __launch_bounds__(256) void k_reorder(
     const float4 x = pos[src];
     pid_s[f] = p;
     xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+    if (xp) reinterpret_cast<Float3P *>(xp)[f] = Float3P{pu.x, pu.y, pu.z};
     xold[f] = make_float4(x.x, x.y, x.z, pu.w);
   }
-  rank_dense_cells(perm, ppid, pred_u, pos, pid_s, xa, xold, dense, dsc, dl);
+  rank_dense_cells(perm, ppid, pred_u, pos, pid_s, xa, xp, xold, dense, dsc,
+                   dl);
 }
 
 // ---------------------------------------------------------------------------
@@ -966,10 +1013,10 @@ struct HitRecorder {
 // for every candidate, whose list entry is that value + j; candidate
 // positions are loaded four at a time so that four loads are in flight per
 // wait.
-template <typename R, typename F>
+template <typename G, typename R, typename F>
 __device__ __forceinline__ void walk_rows(const Scene &sc,
                                           const int *__restrict__ cs,
-                                          const Gather<Pos3> &X, float3 xi,
+                                          const G &X, float3 xi,
                                           R &&on_row, F &&f) {
   int cx, cy, cz;
   cell_coords<kPaper>(sc, xi.x, xi.y, xi.z, cx, cy, cz);
@@ -986,7 +1033,8 @@ __device__ __forceinline__ void walk_rows(const Scene &sc,
       const int re = (row << kOffBits) - b;
       int j = b;
       for (; j + 4 <= e; j += 4) {
-        const Pos3 a0 = X[j], a1 = X[j + 1], a2 = X[j + 2], a3 = X[j + 3];
+        Pos3 a0, a1, a2, a3;
+        X.batch4(j, a0, a1, a2, a3);
         f(j, a0, re);
         f(j + 1, a1, re);
         f(j + 2, a2, re);
@@ -1299,9 +1347,10 @@ struct LambdaSums {
 template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const int *__restrict__ cs,
-    float4 *__restrict__ B, float *__restrict__ rho, uint4 *__restrict__ nl,
-    int *__restrict__ ncnt, int *__restrict__ rbase, int stride) {
+    const float4 *__restrict__ XS, const float *__restrict__ XP,
+    const int *__restrict__ cs, float4 *__restrict__ B,
+    float *__restrict__ rho, uint4 *__restrict__ nl, int *__restrict__ ncnt,
+    int *__restrict__ rbase, int stride) {
   for (Tiles<PT> T(scenes); T.more(); T.next()) {
     const int lb = T.tile();
     const Scene &sc = scenes[scene_of(block_scene, lb)];
@@ -1312,7 +1361,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
     HitRecorder rec{0u, 0u, 0u, 0u, 0, 0, kPerGroup - 1};
     LambdaSums ls;
     const float h = sc.h, h2 = sc.h2;
-    const Gather<Pos3> xsg(XS, stride);
+    const PackedGather xpg(XP, stride);
     // the first candidate slot of every stencil row in the grid, for the
     // replaying passes (row 4, the particle's own, also for the padding);
     // rows outside the grid name no entry and get 0
@@ -1320,7 +1369,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
     unsigned in_grid = 0;
     bool wide = false;
     walk_rows(
-        sc, cs, xsg, xi,
+        sc, cs, xpg, xi,
         [&](int row, int b, int e) {
           rbase[(size_t)row * stride + i] = b;
           in_grid |= 1u << row;
@@ -1939,7 +1988,7 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
             d.count, d.perm, d.ppid, d.dense, d.err);
   PBF_TILES(L, kClsReorder, bs, (k_reorder<true>), (k_reorder<false>), 0,
             d.scenes, d.block_scene, d.perm, d.ppid, d.key, d.cell_start,
-            d.pred_u, d.pos, d.pid_s, d.xa, d.xold, d.dense, d.dsc);
+            d.pred_u, d.pos, d.pid_s, d.xa, d.xp, d.xold, d.dense, d.dsc);
 }
 
 static NextSort next_sort(const DevBuffers &d) {
@@ -1999,8 +2048,8 @@ static void delta_pass(const DevBuffers &d, Launcher &L) {
 
 static void search_pass(const DevBuffers &d, Launcher &L) {
   PBF_GATHER(L, kClsSearch, (k_lambda_paper<true>), (k_lambda_paper<false>),
-             0, d.scenes, d.block_scene, d.xa, d.cell_start, d.xb, d.rho,
-             d.nl, d.ncnt, d.rbase, L.sh.slots);
+             0, d.scenes, d.block_scene, d.xa, d.xp, d.cell_start, d.xb,
+             d.rho, d.nl, d.ncnt, d.rbase, L.sh.slots);
 }
 
 // A lambda pass replaying the lists, over FP32 positions or (Q) over the
@@ -2255,8 +2304,8 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
     k_count_nbrs<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                               d.cell_start, d.nbr);
     k_lambda_paper<false><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
-                                               d.cell_start, d.xb, d.rho,
-                                               d.nl, d.ncnt, d.rbase,
+                                               d.xp, d.cell_start, d.xb,
+                                               d.rho, d.nl, d.ncnt, d.rbase,
                                                sh.slots);
     k_unpack_lambda<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xb,
                                          d.lam);

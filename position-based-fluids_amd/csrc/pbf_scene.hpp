@@ -102,6 +102,8 @@ struct DevBuffers {
                   // count, cursor, then (first slot, end) pairs
   int2 *dsc;      // their scratch (id, slot), one per slot
   float4 *xa;     // x* (sorted): bins, rows and neighbour lists of a step
+  float *xp;      // PAPER: x* (sorted) as 3 floats per slot, the search's
+                  // candidate positions (3 P + 16 floats)
   float4 *xb;     // (x, lambda) of a lambda pass (sorted)
   float4 *xc;     // PAPER: x after a delta pass, .w = rho after the last
   float4 *xold;   // x at step start, m (sorted)
