@@ -361,7 +361,7 @@ void layout_buffers(pbf::DevBuffers &d, const pbf::StepShape &sh,
   take(d.xb, P);
   if (sh.physics == pbf::kPaper) {
     take(d.xp, 3 * P + 16);
-    take(d.xc, P);
+    take(d.xc, 3 * P + 16);
     take(d.xv, P);
   }
   take(d.xold, P);

@@ -174,10 +174,20 @@ __device__ __forceinline__ float3 q_diff(unsigned ax, unsigned ay,
   return make_float3((float)dxy.x, (float)dxy.y, (float)dz);
 }
 
-// x, y, z of a float4 record, read by a 12-byte load at the 16-byte stride
-// (the passes that need only a neighbour's position: the search and the
-// lambda replays; the texture path's time grows with the bytes per lane).
+// x, y, z of a record: a float4 record read by a 12-byte load at the
+// 16-byte stride (Gather<Pos3>), or a packed 12-byte record
+// (Gather<Packed3>), for the passes that need only a neighbour's position.
 struct Pos3 {
+  float x, y, z;
+};
+
+// Positions packed at a 12-byte stride (x* for the search's candidates,
+// x after a delta pass for the lambda replays): more records per cache
+// line, so a wave's gather instruction touches fewer lines.
+struct Packed3 {};
+
+// A 12-byte record at 4-byte alignment (the stores of packed positions).
+struct __attribute__((packed, aligned(4))) Float3P {
   float x, y, z;
 };
 
@@ -187,15 +197,20 @@ typedef unsigned U3 __attribute__((ext_vector_type(3)));
 // of a 64-bit address, and loads past the n records of the array return 0.
 template <typename T>
 struct Gather {
+  static constexpr bool kPos =
+      std::is_same<T, Pos3>::value || std::is_same<T, Packed3>::value;
+  // the value a record is read as
+  using V = std::conditional_t<std::is_same<T, Packed3>::value, Pos3, T>;
   // bytes between records (Pos3 reads float4 records)
-  static constexpr int kStride =
-      std::is_same<T, Pos3>::value ? 16 : (int)sizeof(T);
+  static constexpr int kStride = std::is_same<T, Pos3>::value      ? 16
+                                 : std::is_same<T, Packed3>::value ? 12
+                                                                   : (int)sizeof(T);
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ Gather(const void *p, int n)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
                                             n * kStride, 0x00020000)) {}
-  __device__ __forceinline__ T at_bytes(int off) const {
-    if constexpr (std::is_same<T, Pos3>::value) {
+  __device__ __forceinline__ V at_bytes(int off) const {
+    if constexpr (kPos) {
       const U3 v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
       return Pos3{__uint_as_float(v.x), __uint_as_float(v.y),
                   __uint_as_float(v.z)};
@@ -215,54 +230,44 @@ struct Gather {
           T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
   }
-  __device__ __forceinline__ T operator[](int j) const {
+  __device__ __forceinline__ V operator[](int j) const {
     return at_bytes(j * kStride);
   }
-  // records j .. j + 3, all their loads issued before any is used
-  __device__ __forceinline__ void batch4(int j, T &a0, T &a1, T &a2,
-                                         T &a3) const {
-    a0 = (*this)[j];
-    a1 = (*this)[j + 1];
-    a2 = (*this)[j + 2];
-    a3 = (*this)[j + 3];
+  // records j .. j + 3, all their loads issued before any is used; packed
+  // positions: 48 contiguous bytes, three 16-byte loads (4-byte aligned)
+  __device__ __forceinline__ void batch4(int j, V &a0, V &a1, V &a2,
+                                         V &a3) const {
+    if constexpr (std::is_same<T, Packed3>::value) {
+      const int o = j * 12;
+      const uint4 u = __builtin_bit_cast(
+          uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
+      const uint4 v = __builtin_bit_cast(
+          uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 16, 0, 0));
+      const uint4 w = __builtin_bit_cast(
+          uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 32, 0, 0));
+      a0 = Pos3{__uint_as_float(u.x), __uint_as_float(u.y),
+                __uint_as_float(u.z)};
+      a1 = Pos3{__uint_as_float(u.w), __uint_as_float(v.x),
+                __uint_as_float(v.y)};
+      a2 = Pos3{__uint_as_float(v.z), __uint_as_float(v.w),
+                __uint_as_float(w.x)};
+      a3 = Pos3{__uint_as_float(w.y), __uint_as_float(w.z),
+                __uint_as_float(w.w)};
+    } else {
+      a0 = (*this)[j];
+      a1 = (*this)[j + 1];
+      a2 = (*this)[j + 2];
+      a3 = (*this)[j + 3];
+    }
   }
 };
 
-// x* of the sorted slots as 3 floats each (12-byte records), for the
-// search's candidates: four consecutive candidates are 48 bytes, read by
-// three 16-byte loads (4-byte aligned), which touch fewer cache lines per
-// wave instruction than four loads at the 16-byte stride of x*.  Loads past
-// the n records return 0.
-struct PackedGather {
-  __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ PackedGather(const float *p, int n)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0,
-                                            n * 12, 0x00020000)) {}
-  __device__ __forceinline__ Pos3 operator[](int j) const {
-    const U3 v = __builtin_amdgcn_raw_buffer_load_b96(r, j * 12, 0, 0);
-    return Pos3{__uint_as_float(v.x), __uint_as_float(v.y),
-                __uint_as_float(v.z)};
-  }
-  __device__ __forceinline__ void batch4(int j, Pos3 &a0, Pos3 &a1,
-                                         Pos3 &a2, Pos3 &a3) const {
-    const int o = j * 12;
-    const uint4 u = __builtin_bit_cast(
-        uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
-    const uint4 v = __builtin_bit_cast(
-        uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 16, 0, 0));
-    const uint4 w = __builtin_bit_cast(
-        uint4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 32, 0, 0));
-    a0 = Pos3{__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z)};
-    a1 = Pos3{__uint_as_float(u.w), __uint_as_float(v.x), __uint_as_float(v.y)};
-    a2 = Pos3{__uint_as_float(v.z), __uint_as_float(v.w), __uint_as_float(w.x)};
-    a3 = Pos3{__uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w)};
-  }
-};
-
-// A 12-byte record at 4-byte alignment (x* packed for the search).
-struct __attribute__((packed, aligned(4))) Float3P {
-  float x, y, z;
-};
+// A packed position by a plain load (a particle's own).
+__device__ __forceinline__ float3 packed_at(const float *__restrict__ p,
+                                            int i) {
+  const Float3P v = reinterpret_cast<const Float3P *>(p)[i];
+  return make_float3(v.x, v.y, v.z);
+}
 
 // Cell coordinates of a position.  REF: (int)((x - SYSTEM_MIN) /
 // CELL_LENGTH) with IEEE division (update_cells.cl:9-15).  PAPER: the same
@@ -886,7 +891,7 @@ __device__ __forceinline__ void rank_dense_cells(
       const float4 pu = pred_u[src];
       const float4 x = pos[src];
       pid_s[f] = e.x;
-      xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+      if (xa) xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
       if (xp) reinterpret_cast<Float3P *>(xp)[f] = Float3P{pu.x, pu.y, pu.z};
       xold[f] = make_float4(x.x, x.y, x.z, pu.w);
     }
@@ -927,7 +932,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     const float4 pu = pred_u[src];
     const float4 x = pos[src];
     pid_s[f] = p;
-    xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
+    if (xa) xa[f] = make_float4(pu.x, pu.y, pu.z, 0.0f);
     if (xp) reinterpret_cast<Float3P *>(xp)[f] = Float3P{pu.x, pu.y, pu.z};
     xold[f] = make_float4(x.x, x.y, x.z, pu.w);
   }
@@ -1063,7 +1068,7 @@ __device__ __forceinline__ unsigned group_entry(const unsigned (&ws)[4],
   return __builtin_amdgcn_alignbit(ws[wi + 1], ws[wi], sh);
 }
 
-// f(j, row, X[j]) for every j of N_i in recorded order, the entries of a
+// f(X[j]) for every j of N_i in recorded order, the entries of a
 // group resolved and all their loads issued before any is used (more loads
 // in flight per wave measured faster than fewer VGPRs and more waves); the
 // padding of the last group names i itself, which every pass's 0 < r^2
@@ -1071,55 +1076,53 @@ __device__ __forceinline__ unsigned group_entry(const unsigned (&ws)[4],
 // rb is LDS scratch of 9 x blockDim.x ints.
 template <typename T, typename F>
 __device__ __forceinline__ void for_neighbours(
-    const Scene &sc, const int *__restrict__ cs, const Gather<Pos3> &XS,
+    const Scene &sc, const int *__restrict__ cs, const Gather<Packed3> &XS,
     const Gather<T> &X, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, const int *__restrict__ rbase, int stride,
     int i, int *rb, F &&f) {
-  constexpr int kShift =
-      Gather<T>::kStride == 32 ? 5 : Gather<T>::kStride == 16 ? 4 : 3;
+  constexpr int kS = Gather<T>::kStride;
   const int cnt = ncnt[i];
   if (cnt <= kMaxHits) {
-    // rb[row * bs + lane] = the row's start (lanes of one row in
-    // consecutive banks), from the starts the search stored
+    // rb[row * bs + lane] = the row's start in bytes of X (lanes of one row
+    // in consecutive banks), from the starts the search stored
     const unsigned bs = blockDim.x;
     int *my = rb + threadIdx.x;
 #pragma unroll
-    for (int r = 0; r < 9; ++r) my[r * bs] = rbase[(size_t)r * stride + i];
+    for (int r = 0; r < 9; ++r)
+      my[r * bs] = rbase[(size_t)r * stride + i] * kS;
     // table address row * 4 bs + 4 lane (one 24-bit multiply-add); record
-    // byte offset (start + entry offset) << shift (one add-shift)
+    // byte offset start + entry offset * stride (one shift-add or 24-bit
+    // multiply-add)
     const unsigned bs4 = bs * 4u, lane4 = threadIdx.x * 4u;
     const char *rbb = reinterpret_cast<const char *>(rb);
     constexpr unsigned kOffMask = (1u << kOffBits) - 1u;
     for (int g = 0; g * kPerGroup < cnt; ++g) {
       const uint4 w = nl[(size_t)g * stride + i];
       const unsigned ws[4] = {w.x, w.y, w.z, w.w};
-      int off[kPerGroup], rows[kPerGroup];
+      int off[kPerGroup];
 #pragma unroll
       for (int q = 0; q < kPerGroup; ++q) {
         const unsigned e = group_entry(ws, q);
         const unsigned row = (e >> kOffBits) & 0xfu;
-        rows[q] = (int)row;
-        off[q] = (*reinterpret_cast<const int *>(rbb + row * bs4 + lane4) +
-                  (int)(e & kOffMask))
-                 << kShift;
+        off[q] = *reinterpret_cast<const int *>(rbb + row * bs4 + lane4) +
+                 (int)(e & kOffMask) * kS;
       }
-      T xv[kPerGroup];
+      typename Gather<T>::V xv[kPerGroup];
 #pragma unroll
       for (int q = 0; q < kPerGroup; ++q) xv[q] = X.at_bytes(off[q]);
 #pragma unroll
-      for (int q = 0; q < kPerGroup; ++q) f(off[q] >> kShift, rows[q], xv[q]);
+      for (int q = 0; q < kPerGroup; ++q) f(xv[q]);
     }
   } else {
     const float h2 = sc.h2;
     const Pos3 s4 = XS[i];
     const float3 xs = make_float3(s4.x, s4.y, s4.z);
-    int row = 0;
     walk_rows(
-        sc, cs, XS, xs, [&](int r, int, int) { row = r; },
+        sc, cs, XS, xs, [](int, int, int) {},
         [&](int j, Pos3 sj, int) {
           const float rx = xs.x - sj.x, ry = xs.y - sj.y, rz = xs.z - sj.z;
           const float r2 = rx * rx + ry * ry + rz * rz;
-          if (in_reach(r2, h2)) f(j, row, X[j]);
+          if (in_reach(r2, h2)) f(X[j]);
         });
   }
 }
@@ -1283,7 +1286,7 @@ __global__ __launch_bounds__(256) void k_xsph_final_ref(
 // Buffers: XS = x* (sorted; it fixes the bins, rows and neighbour lists of
 // the step and is not written after the sort), B = (x, lambda) written by a
 // lambda pass so that the delta pass reads one float4 per neighbour, and
-// XC = x after a delta pass (.w = rho after the last one).  Iteration k's
+// XC = x after a delta pass (packed, 12 bytes per slot).  Iteration k's
 // lambda pass reads XS (k = 0, the search) or XC; its delta pass reads B and
 // writes XC.
 // ---------------------------------------------------------------------------
@@ -1347,8 +1350,8 @@ struct LambdaSums {
 template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const float *__restrict__ XP,
-    const int *__restrict__ cs, float4 *__restrict__ B,
+    const float *__restrict__ XP, const int *__restrict__ cs,
+    float4 *__restrict__ B,
     float *__restrict__ rho, uint4 *__restrict__ nl, int *__restrict__ ncnt,
     int *__restrict__ rbase, int stride) {
   for (Tiles<PT> T(scenes); T.more(); T.next()) {
@@ -1356,12 +1359,11 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
     const Scene &sc = scenes[scene_of(block_scene, lb)];
     const int i = lb * blockDim.x + threadIdx.x;
     if (i - sc.slot_off >= sc.n) continue;
-    const float4 x4 = XS[i];
-    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    const float3 xi = packed_at(XP, i);
     HitRecorder rec{0u, 0u, 0u, 0u, 0, 0, kPerGroup - 1};
     LambdaSums ls;
     const float h = sc.h, h2 = sc.h2;
-    const PackedGather xpg(XP, stride);
+    const Gather<Packed3> xpg(XP, stride);
     // the first candidate slot of every stencil row in the grid, for the
     // replaying passes (row 4, the particle's own, also for the padding);
     // rows outside the grid name no entry and get 0
@@ -1396,7 +1398,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
 template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_list(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
+    const float *__restrict__ XS, const float *__restrict__ XC,
     const int *__restrict__ cs, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, float4 *__restrict__ B,
@@ -1407,13 +1409,13 @@ __global__ __launch_bounds__(256) void k_lambda_list(
     const Scene &sc = scenes[scene_of(block_scene, lb)];
     const int i = lb * blockDim.x + threadIdx.x;
     if (i - sc.slot_off >= sc.n) continue;
-    const float4 x4 = XC[i];
-    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    const float3 xi = packed_at(XC, i);
     LambdaSums ls;
     const float h = sc.h, h2 = sc.h2;
-    const Gather<Pos3> xsg(XS, stride), xcg(XC, stride);
+    const Gather<Packed3> xsg(XS, stride);
+    const Gather<Packed3> xcg(XC, stride);
     for_neighbours(sc, cs, xsg, xcg, nl, ncnt, rbase, stride, i, rb_lds,
-                   [&](int, int, Pos3 xj) {
+                   [&](Pos3 xj) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       ls.add_if(in_reach(r2, h2), h, h2, rx, ry, rz, r2);
@@ -1432,7 +1434,7 @@ constexpr float kQH2 = kQH * kQH;
 template <bool PT>
 __global__ __launch_bounds__(256) void k_lambda_list_q(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const float4 *__restrict__ XC,
+    const float *__restrict__ XS, const float *__restrict__ XC,
     const uint2 *__restrict__ QA, const int *__restrict__ cs,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, float4 *__restrict__ B,
@@ -1443,16 +1445,15 @@ __global__ __launch_bounds__(256) void k_lambda_list_q(
     const Scene &sc = scenes[scene_of(block_scene, lb)];
     const int i = lb * blockDim.x + threadIdx.x;
     if (i - sc.slot_off >= sc.n) continue;
-    const float4 x4 = XC[i];
-    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    const float3 xi = packed_at(XC, i);
     // the own record, quantised as the neighbours' are (the padding entries,
     // which name i itself, then give r = 0 exactly)
     const uint2 qi = QA[i];
     LambdaSums ls;
-    const Gather<Pos3> xsg(XS, stride);
+    const Gather<Packed3> xsg(XS, stride);
     const Gather<uint2> qag(QA, stride);
     for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
-                   [&](int, int, uint2 qj) {
+                   [&](uint2 qj) {
       const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
       const float r2 = r.x * r.x + r.y * r.y + r.z * r.z;
       ls.add_if(in_reach(r2, kQH2), kQH, kQH2, r.x, r.y, r.z, r2);
@@ -1468,11 +1469,11 @@ __global__ __launch_bounds__(256) void k_lambda_list_q(
 template <bool PT, bool LAST, bool WRITE_QA, bool N4, bool WRITE_QV>
 __global__ __launch_bounds__(256) void k_delta_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const float4 *__restrict__ B,
+    const float *__restrict__ XS, const float4 *__restrict__ B,
     const int *__restrict__ cs, const float *__restrict__ rho,
     const float4 *__restrict__ xold, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, const int *__restrict__ rbase, int stride,
-    float4 *__restrict__ XC, XVRec *__restrict__ XV, uint2 *__restrict__ QA,
+    float *__restrict__ XC, XVRec *__restrict__ XV, uint2 *__restrict__ QA,
     uint4 *__restrict__ QV) {
   extern __shared__ int rb_lds[];
   for (Tiles<PT> T(scenes); T.more(); T.next()) {
@@ -1486,14 +1487,14 @@ __global__ __launch_bounds__(256) void k_delta_paper(
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     const float h = sc.h, h2 = sc.h2, cq = sc.scorr_cq, kk = sc.scorr_k;
     const int n = sc.scorr_n;
-    const Gather<Pos3> xsg(XS, stride);
+    const Gather<Packed3> xsg(XS, stride);
     const Gather<float4> bg(B, stride);
     // (lambda_i + lambda_j + s_corr) (h-r)^2 r/|r| summed branch-free (the
     // out-of-reach entries selected to 0).  N4: the paper's exponent n = 4 in
     // every scene (the host picks the instance; the loop over any n costs
     // registers and occupancy)
     for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
-                   [&](int, int, float4 xj) {
+                   [&](float4 xj) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = rx * rx + ry * ry + rz * rz;
       const float inv = __builtin_amdgcn_rsqf(r2);
@@ -1526,7 +1527,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
                     make_float4(v.x, v.y, v.z, xo.w)};
       if constexpr (WRITE_QV) QV[i] = pack_qv(sc, xn, rho[i], v);
     } else {
-      XC[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+      reinterpret_cast<Float3P *>(XC)[i] = Float3P{xn.x, xn.y, xn.z};
     }
   }
 }
@@ -1539,7 +1540,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
 // by q_unit^6 / rho0, since the records hold rho0 / rho; om by q_unit^2).
 template <bool QIN, bool OMEGA>
 __device__ __forceinline__ void xsph_sums(
-    const Scene &sc, const int *__restrict__ cs, const Gather<Pos3> &xsg,
+    const Scene &sc, const int *__restrict__ cs, const Gather<Packed3> &xsg,
     const XVRec *__restrict__ XV, const uint4 *__restrict__ QV,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, int i, int *rb, float3 xi,
@@ -1571,7 +1572,7 @@ __device__ __forceinline__ void xsph_sums(
     const uint4 qi = QV[i];
     const Gather<uint4> qvg(QV, stride);
     for_neighbours(sc, cs, xsg, qvg, nl, ncnt, rbase, stride, i, rb,
-                   [&](int, int, uint4 qj) {
+                   [&](uint4 qj) {
       const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
       term(r.x, r.y, r.z, f16_hi(qj.y),
            make_float3(f16_lo(qj.z), f16_hi(qj.z), f16_lo(qj.w)));
@@ -1579,7 +1580,7 @@ __device__ __forceinline__ void xsph_sums(
   } else {
     const Gather<XVRec> xvg(XV, stride);
     for_neighbours(sc, cs, xsg, xvg, nl, ncnt, rbase, stride, i, rb,
-                   [&](int, int, XVRec nb) {
+                   [&](XVRec nb) {
       term(xi.x - nb.x.x, xi.y - nb.x.y, xi.z - nb.x.z, nb.x.w,
            make_float3(nb.v.x, nb.v.y, nb.v.z));
     });
@@ -1603,7 +1604,7 @@ __device__ __forceinline__ float xsph_factor(const Scene &sc) {
 template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
+    const float *__restrict__ XS, const XVRec *__restrict__ XV,
     const uint4 *__restrict__ QV, const int *__restrict__ cs,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride,
@@ -1620,7 +1621,7 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
     const float4 vi = own.v;
     float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
-    const Gather<Pos3> xsg(XS, stride);
+    const Gather<Packed3> xsg(XS, stride);
     xsph_sums<QIN, false>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
                           rb_lds, xi, vi, s, om);
     const float c = xsph_factor<QIN>(sc);
@@ -1643,7 +1644,7 @@ constexpr float kOmegaScale = 1.0f / 256.0f;
 template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const XVRec *__restrict__ XV,
+    const float *__restrict__ XS, const XVRec *__restrict__ XV,
     const uint4 *__restrict__ QV, const int *__restrict__ cs,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, float4 *__restrict__ VX,
@@ -1659,7 +1660,7 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
     const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
     const float4 vi = own.v;
     float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
-    const Gather<Pos3> xsg(XS, stride);
+    const Gather<Packed3> xsg(XS, stride);
     xsph_sums<QIN, true>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
                          rb_lds, xi, vi, s, om);
     const float cx = xsph_factor<QIN>(sc);
@@ -1684,7 +1685,7 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
 template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_vort_final(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float4 *__restrict__ XS, const float4 *__restrict__ B,
+    const float *__restrict__ XS, const float4 *__restrict__ B,
     const XVRec *__restrict__ XV, const uint2 *__restrict__ QA,
     const int *__restrict__ cs, const float4 *__restrict__ VX,
     const float4 *__restrict__ OM, const uint4 *__restrict__ nl,
@@ -1712,21 +1713,21 @@ __global__ __launch_bounds__(256) void k_vort_final(
       ey += ry * gs;
       ez += rz * gs;
     };
-    const Gather<Pos3> xsg(XS, stride);
+    const Gather<Packed3> xsg(XS, stride);
     if constexpr (QIN) {
       // |w_i| rounded as the neighbours' are, so that equal magnitudes cancel
       const uint2 qi = QA[i];
       const float wq = f16_hi(qi.y);
       const Gather<uint2> qag(QA, stride);
       for_neighbours(sc, cs, xsg, qag, nl, ncnt, rbase, stride, i, rb_lds,
-                     [&](int, int, uint2 qj) {
+                     [&](uint2 qj) {
         const float3 r = q_diff(qi.x, qi.y, qj.x, qj.y);
         term(r.x, r.y, r.z, f16_hi(qj.y) - wq);
       });
     } else {
       const Gather<float4> bg(B, stride);
       for_neighbours(sc, cs, xsg, bg, nl, ncnt, rbase, stride, i, rb_lds,
-                     [&](int, int, float4 xj) {
+                     [&](float4 xj) {
         term(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, xj.w - wi.w);
       });
     }
@@ -1986,9 +1987,13 @@ static void sort_stage(const DevBuffers &d, Launcher &L, int predict,
   PBF_TILES(L, kClsScatter, bs, (k_scatter<true>), (k_scatter<false>), 0,
             d.scenes, d.block_scene, d.key, d.rank, d.pid, d.cell_start,
             d.count, d.perm, d.ppid, d.dense, d.err);
+  // x* as float4 (REF, and the probe and diagnostics of either physics)
+  // and packed (PAPER)
+  float4 *xa = PH == kRef || with_predict ? d.xa : nullptr;
+  float *xp = PH == kPaper ? d.xp : nullptr;
   PBF_TILES(L, kClsReorder, bs, (k_reorder<true>), (k_reorder<false>), 0,
             d.scenes, d.block_scene, d.perm, d.ppid, d.key, d.cell_start,
-            d.pred_u, d.pos, d.pid_s, d.xa, d.xp, d.xold, d.dense, d.dsc);
+            d.pred_u, d.pos, d.pid_s, xa, xp, d.xold, d.dense, d.dsc);
 }
 
 static NextSort next_sort(const DevBuffers &d) {
@@ -2036,19 +2041,19 @@ static void delta_pass(const DevBuffers &d, Launcher &L) {
     PBF_GATHER(L, kClsDelta,
                (k_delta_paper<true, LAST, WRITE_QA, true, WRITE_QV>),
                (k_delta_paper<false, LAST, WRITE_QA, true, WRITE_QV>), 9,
-               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+               d.scenes, d.block_scene, d.xp, d.xb, d.cell_start, d.rho,
                d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
   else
     PBF_GATHER(L, kClsDelta,
                (k_delta_paper<true, LAST, WRITE_QA, false, WRITE_QV>),
                (k_delta_paper<false, LAST, WRITE_QA, false, WRITE_QV>), 9,
-               d.scenes, d.block_scene, d.xa, d.xb, d.cell_start, d.rho,
+               d.scenes, d.block_scene, d.xp, d.xb, d.cell_start, d.rho,
                d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
 }
 
 static void search_pass(const DevBuffers &d, Launcher &L) {
   PBF_GATHER(L, kClsSearch, (k_lambda_paper<true>), (k_lambda_paper<false>),
-             0, d.scenes, d.block_scene, d.xa, d.xp, d.cell_start, d.xb,
+             0, d.scenes, d.block_scene, d.xp, d.cell_start, d.xb,
              d.rho, d.nl, d.ncnt, d.rbase, L.sh.slots);
 }
 
@@ -2059,12 +2064,12 @@ static void lambda_list_pass(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
   if constexpr (Q)
     PBF_GATHER(L, kClsLambda, (k_lambda_list_q<true>),
-               (k_lambda_list_q<false>), 9, d.scenes, d.block_scene, d.xa,
+               (k_lambda_list_q<false>), 9, d.scenes, d.block_scene, d.xp,
                d.xc, d.qa, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.xb,
                d.rho);
   else
     PBF_GATHER(L, kClsLambda, (k_lambda_list<true>), (k_lambda_list<false>),
-               9, d.scenes, d.block_scene, d.xa, d.xc, d.cell_start, d.nl,
+               9, d.scenes, d.block_scene, d.xp, d.xc, d.cell_start, d.nl,
                d.ncnt, d.rbase, P, d.xb, d.rho);
 }
 
@@ -2074,10 +2079,10 @@ template <bool Q>
 static void vorticity_passes(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
   PBF_GATHER(L, kClsXsph, (k_xsph_omega<true, Q>), (k_xsph_omega<false, Q>),
-             9, d.scenes, d.block_scene, d.xa, d.xv, d.qv, d.cell_start, d.nl,
+             9, d.scenes, d.block_scene, d.xp, d.xv, d.qv, d.cell_start, d.nl,
              d.ncnt, d.rbase, P, d.vx, d.om, d.xb, d.qa);
   PBF_GATHER(L, kClsVort, (k_vort_final<true, Q>), (k_vort_final<false, Q>),
-             9, d.scenes, d.block_scene, d.xa, d.xb, d.xv, d.qa, d.cell_start,
+             9, d.scenes, d.block_scene, d.xp, d.xb, d.xv, d.qa, d.cell_start,
              d.vx, d.om, d.nl, d.ncnt, d.rbase, P, d.pid_s, d.pos, d.vel,
              d.pid, d.err, next_sort(d));
 }
@@ -2088,7 +2093,7 @@ template <bool Q>
 static void xsph_final_pass(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
   PBF_GATHER(L, kClsXsph, (k_xsph_final_paper<true, Q>),
-             (k_xsph_final_paper<false, Q>), 9, d.scenes, d.block_scene, d.xa,
+             (k_xsph_final_paper<false, Q>), 9, d.scenes, d.block_scene, d.xp,
              d.xv, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.pid_s,
              d.pos, d.vel, d.pid, d.err, next_sort(d));
 }
@@ -2303,9 +2308,9 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
     sort_stage<kPaper>(d, L, 1, true);
     k_count_nbrs<kPaper><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                               d.cell_start, d.nbr);
-    k_lambda_paper<false><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
-                                               d.xp, d.cell_start, d.xb,
-                                               d.rho, d.nl, d.ncnt, d.rbase,
+    k_lambda_paper<false><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xp,
+                                               d.cell_start, d.xb, d.rho,
+                                               d.nl, d.ncnt, d.rbase,
                                                sh.slots);
     k_unpack_lambda<<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xb,
                                          d.lam);

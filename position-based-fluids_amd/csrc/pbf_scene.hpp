@@ -105,7 +105,8 @@ struct DevBuffers {
   float *xp;      // PAPER: x* (sorted) as 3 floats per slot, the search's
                   // candidate positions (3 P + 16 floats)
   float4 *xb;     // (x, lambda) of a lambda pass (sorted)
-  float4 *xc;     // PAPER: x after a delta pass, .w = rho after the last
+  float *xc;      // PAPER: x after a delta pass but the last, packed (3 P
+                  // + 16 floats), for the next lambda pass
   float4 *xold;   // x at step start, m (sorted)
   float *lam;     // lambda (sorted)
   float *rho;     // density of the last lambda pass (sorted)
