@@ -362,7 +362,7 @@ void layout_buffers(pbf::DevBuffers &d, const pbf::StepShape &sh,
   if (sh.physics == pbf::kPaper) {
     take(d.xp, 3 * P + 16);
     take(d.xc, 3 * P + 16);
-    take(d.xv, P);
+    take(d.vr, P);
   }
   take(d.xold, P);
   take(d.lam, P);

@@ -262,6 +262,35 @@ struct Gather {
   }
 };
 
+// XSPH's neighbour record from two arrays: x from the packed positions (12
+// bytes per slot) and (v, 1/rho) (16 bytes per slot), two loads that touch
+// fewer cache lines than one 32-byte record's two halves.  Offsets are in
+// units of 4 bytes per slot.
+struct PosVel {};
+
+template <>
+struct Gather<PosVel> {
+  using V = XVRec;
+  static constexpr int kStride = 4;
+  __amdgpu_buffer_rsrc_t rx, rv;
+  __device__ __forceinline__ Gather(const float *x, const float4 *v, int n)
+      : rx(__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(x), (short)0,
+                                             n * 12, 0x00020000)),
+        rv(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(v),
+                                             (short)0, n * 16, 0x00020000)) {}
+  __device__ __forceinline__ V at_bytes(int off) const {
+    const U3 p = __builtin_amdgcn_raw_buffer_load_b96(rx, off * 3, 0, 0);
+    const float4 q = __builtin_bit_cast(
+        float4, __builtin_amdgcn_raw_buffer_load_b128(rv, off * 4, 0, 0));
+    return XVRec{make_float4(__uint_as_float(p.x), __uint_as_float(p.y),
+                             __uint_as_float(p.z), q.w),
+                 make_float4(q.x, q.y, q.z, 0.0f)};
+  }
+  __device__ __forceinline__ V operator[](int j) const {
+    return at_bytes(j * kStride);
+  }
+};
+
 // A packed position by a plain load (a particle's own).
 __device__ __forceinline__ float3 packed_at(const float *__restrict__ p,
                                             int i) {
@@ -1462,8 +1491,8 @@ __global__ __launch_bounds__(256) void k_lambda_list_q(
   }
 }
 
-// eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST writes
-// XV = ((x, rho), (v, m)) instead, v = (x - x_n)/dt, for the XSPH pass.
+// eqs. 13, 14 and the collision clamp over N_i; writes XC.  LAST also
+// writes VR = (v, 1/rho), v = (x - x_n)/dt, for the XSPH pass.
 // WRITE_QA / WRITE_QV (mixed precision) also write the 16-bit records of
 // the next lambda pass / of the XSPH pass.
 template <bool PT, bool LAST, bool WRITE_QA, bool N4, bool WRITE_QV>
@@ -1473,7 +1502,7 @@ __global__ __launch_bounds__(256) void k_delta_paper(
     const int *__restrict__ cs, const float *__restrict__ rho,
     const float4 *__restrict__ xold, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt, const int *__restrict__ rbase, int stride,
-    float *__restrict__ XC, XVRec *__restrict__ XV, uint2 *__restrict__ QA,
+    float *__restrict__ XC, float4 *__restrict__ VR, uint2 *__restrict__ QA,
     uint4 *__restrict__ QV) {
   extern __shared__ int rb_lds[];
   for (Tiles<PT> T(scenes); T.more(); T.next()) {
@@ -1520,14 +1549,11 @@ __global__ __launch_bounds__(256) void k_delta_paper(
     const float3 dp = make_float3(ax * c, ay * c, az * c);
     const float3 xn = clamp_update(sc, xi, dp, sc.sphere[3] > 0.0f);
     if constexpr (WRITE_QA) QA[i] = pack_q(sc, xn, 0u);
+    reinterpret_cast<Float3P *>(XC)[i] = Float3P{xn.x, xn.y, xn.z};
     if constexpr (LAST) {
-      const float4 xo = xold[i];
-      const float3 v = velocity_exact(sc, xn, xo);
-      XV[i] = XVRec{make_float4(xn.x, xn.y, xn.z, inv_density(rho[i])),
-                    make_float4(v.x, v.y, v.z, xo.w)};
+      const float3 v = velocity_exact(sc, xn, xold[i]);
+      VR[i] = make_float4(v.x, v.y, v.z, inv_density(rho[i]));
       if constexpr (WRITE_QV) QV[i] = pack_qv(sc, xn, rho[i], v);
-    } else {
-      reinterpret_cast<Float3P *>(XC)[i] = Float3P{xn.x, xn.y, xn.z};
     }
   }
 }
@@ -1541,7 +1567,8 @@ __global__ __launch_bounds__(256) void k_delta_paper(
 template <bool QIN, bool OMEGA>
 __device__ __forceinline__ void xsph_sums(
     const Scene &sc, const int *__restrict__ cs, const Gather<Packed3> &xsg,
-    const XVRec *__restrict__ XV, const uint4 *__restrict__ QV,
+    const float *__restrict__ XC, const float4 *__restrict__ VR,
+    const uint4 *__restrict__ QV,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, int i, int *rb, float3 xi,
     float4 vi, float3 &s, float3 &om) {
@@ -1578,7 +1605,7 @@ __device__ __forceinline__ void xsph_sums(
            make_float3(f16_lo(qj.z), f16_hi(qj.z), f16_lo(qj.w)));
     });
   } else {
-    const Gather<XVRec> xvg(XV, stride);
+    const Gather<PosVel> xvg(XC, VR, stride);
     for_neighbours(sc, cs, xsg, xvg, nl, ncnt, rbase, stride, i, rb,
                    [&](XVRec nb) {
       term(xi.x - nb.x.x, xi.y - nb.x.y, xi.z - nb.x.z, nb.x.w,
@@ -1604,7 +1631,8 @@ __device__ __forceinline__ float xsph_factor(const Scene &sc) {
 template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float *__restrict__ XS, const XVRec *__restrict__ XV,
+    const float *__restrict__ XS, const float *__restrict__ XC,
+    const float4 *__restrict__ VR, const float4 *__restrict__ xold,
     const uint4 *__restrict__ QV, const int *__restrict__ cs,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride,
@@ -1617,12 +1645,13 @@ __global__ __launch_bounds__(256) void k_xsph_final_paper(
     const Scene &sc = scenes[scene_of(block_scene, lb)];
     const int i = lb * blockDim.x + threadIdx.x;
     if (i - sc.slot_off >= sc.n) continue;
-    const XVRec own = XV[i];
-    const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
-    const float4 vi = own.v;
+    const float3 xi = packed_at(XC, i);
+    // own v and m (the mass from x_n's record)
+    const float4 vr = VR[i];
+    const float4 vi = make_float4(vr.x, vr.y, vr.z, xold[i].w);
     float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
     const Gather<Packed3> xsg(XS, stride);
-    xsph_sums<QIN, false>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
+    xsph_sums<QIN, false>(sc, cs, xsg, XC, VR, QV, nl, ncnt, rbase, stride, i,
                           rb_lds, xi, vi, s, om);
     const float c = xsph_factor<QIN>(sc);
     const float4 v =
@@ -1644,7 +1673,8 @@ constexpr float kOmegaScale = 1.0f / 256.0f;
 template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
-    const float *__restrict__ XS, const XVRec *__restrict__ XV,
+    const float *__restrict__ XS, const float *__restrict__ XC,
+    const float4 *__restrict__ VR, const float4 *__restrict__ xold,
     const uint4 *__restrict__ QV, const int *__restrict__ cs,
     const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
     const int *__restrict__ rbase, int stride, float4 *__restrict__ VX,
@@ -1656,12 +1686,13 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
     const Scene &sc = scenes[scene_of(block_scene, lb)];
     const int i = lb * blockDim.x + threadIdx.x;
     if (i - sc.slot_off >= sc.n) continue;
-    const XVRec own = XV[i];
-    const float3 xi = make_float3(own.x.x, own.x.y, own.x.z);
-    const float4 vi = own.v;
+    const float3 xi = packed_at(XC, i);
+    // own v and m (the mass from x_n's record)
+    const float4 vr = VR[i];
+    const float4 vi = make_float4(vr.x, vr.y, vr.z, xold[i].w);
     float3 s = make_float3(0.0f, 0.0f, 0.0f), om = s;
     const Gather<Packed3> xsg(XS, stride);
-    xsph_sums<QIN, true>(sc, cs, xsg, XV, QV, nl, ncnt, rbase, stride, i,
+    xsph_sums<QIN, true>(sc, cs, xsg, XC, VR, QV, nl, ncnt, rbase, stride, i,
                          rb_lds, xi, vi, s, om);
     const float cx = xsph_factor<QIN>(sc);
     VX[i] =
@@ -1681,12 +1712,12 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
 // v += dt eps (N x w_i) (eq. 16), then update_positions.cl; reads
 // B = (x, |omega|), or (QIN) the 16-bit records QA (eta formed in units of
 // q_unit and of the FP16 scale: only its direction is used) and the own
-// position from XV.
+// position from XC.
 template <bool PT, bool QIN>
 __global__ __launch_bounds__(256) void k_vort_final(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float *__restrict__ XS, const float4 *__restrict__ B,
-    const XVRec *__restrict__ XV, const uint2 *__restrict__ QA,
+    const float *__restrict__ XC, const uint2 *__restrict__ QA,
     const int *__restrict__ cs, const float4 *__restrict__ VX,
     const float4 *__restrict__ OM, const uint4 *__restrict__ nl,
     const int *__restrict__ ncnt,
@@ -1699,8 +1730,13 @@ __global__ __launch_bounds__(256) void k_vort_final(
     const Scene &sc = scenes[scene_of(block_scene, lb)];
     const int i = lb * blockDim.x + threadIdx.x;
     if (i - sc.slot_off >= sc.n) continue;
-    const float4 x4 = QIN ? XV[i].x : B[i];
-    const float3 xi = make_float3(x4.x, x4.y, x4.z);
+    float3 xi;
+    if constexpr (QIN) {
+      xi = packed_at(XC, i);
+    } else {
+      const float4 x4 = B[i];
+      xi = make_float3(x4.x, x4.y, x4.z);
+    }
     const float4 wi = OM[i];
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
     const float h = QIN ? kQH : sc.h, h2 = QIN ? kQH2 : sc.h2;
@@ -2042,13 +2078,13 @@ static void delta_pass(const DevBuffers &d, Launcher &L) {
                (k_delta_paper<true, LAST, WRITE_QA, true, WRITE_QV>),
                (k_delta_paper<false, LAST, WRITE_QA, true, WRITE_QV>), 9,
                d.scenes, d.block_scene, d.xp, d.xb, d.cell_start, d.rho,
-               d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
+               d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.vr, d.qa, d.qv);
   else
     PBF_GATHER(L, kClsDelta,
                (k_delta_paper<true, LAST, WRITE_QA, false, WRITE_QV>),
                (k_delta_paper<false, LAST, WRITE_QA, false, WRITE_QV>), 9,
                d.scenes, d.block_scene, d.xp, d.xb, d.cell_start, d.rho,
-               d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.xv, d.qa, d.qv);
+               d.xold, d.nl, d.ncnt, d.rbase, P, d.xc, d.vr, d.qa, d.qv);
 }
 
 static void search_pass(const DevBuffers &d, Launcher &L) {
@@ -2079,10 +2115,10 @@ template <bool Q>
 static void vorticity_passes(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
   PBF_GATHER(L, kClsXsph, (k_xsph_omega<true, Q>), (k_xsph_omega<false, Q>),
-             9, d.scenes, d.block_scene, d.xp, d.xv, d.qv, d.cell_start, d.nl,
-             d.ncnt, d.rbase, P, d.vx, d.om, d.xb, d.qa);
+             9, d.scenes, d.block_scene, d.xp, d.xc, d.vr, d.xold, d.qv,
+             d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx, d.om, d.xb, d.qa);
   PBF_GATHER(L, kClsVort, (k_vort_final<true, Q>), (k_vort_final<false, Q>),
-             9, d.scenes, d.block_scene, d.xp, d.xb, d.xv, d.qa, d.cell_start,
+             9, d.scenes, d.block_scene, d.xp, d.xb, d.xc, d.qa, d.cell_start,
              d.vx, d.om, d.nl, d.ncnt, d.rbase, P, d.pid_s, d.pos, d.vel,
              d.pid, d.err, next_sort(d));
 }
@@ -2094,7 +2130,8 @@ static void xsph_final_pass(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
   PBF_GATHER(L, kClsXsph, (k_xsph_final_paper<true, Q>),
              (k_xsph_final_paper<false, Q>), 9, d.scenes, d.block_scene, d.xp,
-             d.xv, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P, d.pid_s,
+             d.xc, d.vr, d.xold, d.qv, d.cell_start, d.nl, d.ncnt, d.rbase, P,
+             d.pid_s,
              d.pos, d.vel, d.pid, d.err, next_sort(d));
 }
 

@@ -36,7 +36,7 @@ constexpr int kDenseCell = 256;
 constexpr int kDenseBuckets = 1024;
 
 // pbf_step_noalloc: bytes of the static device scratch (per GPU) that
-// holds every temporary of the call (at 436 bytes per particle slot plus 8
+// holds every temporary of the call (at 420 bytes per particle slot plus 8
 // per cell: the 32,768-particle dam break in PAPER physics uses 14 MB).
 constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
 

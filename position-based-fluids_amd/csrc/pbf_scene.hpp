@@ -73,11 +73,11 @@ struct Scene {
 };
 static_assert(sizeof(Scene) % 16 == 0, "Scene must stay 16-byte aligned");
 
-// Position and velocity of one slot side by side, so that a gather of a
-// neighbour's pair reads one 32-byte record.
+// A neighbour's record for XSPH as the gather returns it: x (from the packed
+// positions) with 1/rho, and v, read from two arrays.
 struct XVRec {
   float4 x;  // x, 1/rho (0: no density)
-  float4 v;  // v, m
+  float4 v;  // v, 0
 };
 
 // Device buffers of one handle.  Slot-indexed arrays have P entries (all
@@ -111,7 +111,8 @@ struct DevBuffers {
   float *lam;     // lambda (sorted)
   float *rho;     // density of the last lambda pass (sorted)
   float4 *v;      // REF: velocity after the constraint solve (sorted), .w = m
-  XVRec *xv;      // PAPER: (x, 1/rho), (v, m) after the last delta pass
+  float4 *vr;     // PAPER: (v, 1/rho) after the last delta pass (its x in
+                  // xc), for XSPH
   float4 *vx;     // PAPER: v + XSPH (sorted), .w = m
   float4 *om;     // PAPER: vorticity, .w = |omega| (sorted)
   // mixed precision records (sorted), 16-bit fixed-point x (x | y << 16,
