@@ -848,8 +848,12 @@ __global__ __launch_bounds__(256) void k_scatter(
 // the cells k_scatter listed from a cursor once their tiles are done, until
 // the list is exhausted; writes what the tiles write.  Every work-item of
 // the work-group calls it (work-group barriers).
+// One table of kDenseBuckets ints serves the three phases of a cell (the
+// counts, scanned in place into the bucket starts, then advanced as the
+// scatter's cursors, after which bucket b spans [cur[b - 1], cur[b])), so
+// the LDS a work-group holds does not limit the reorder's occupancy.
 struct DenseLds {
-  int cnt[kDenseBuckets], start[kDenseBuckets], cur[kDenseBuckets];
+  int cur[kDenseBuckets];
   int item, min_id, max_id;
 };
 
@@ -874,7 +878,7 @@ __device__ __forceinline__ void rank_dense_cells(
     __syncthreads();
     const int item = L.item;
     if (item >= ncell) return;  // uniform: every work-group reaches it
-    for (int b = tid; b < kDenseBuckets; b += nt) L.cnt[b] = L.cur[b] = 0;
+    for (int b = tid; b < kDenseBuckets; b += nt) L.cur[b] = 0;
     if (tid == 0) {
       L.min_id = INT_MAX;
       L.max_id = INT_MIN;
@@ -892,28 +896,30 @@ __device__ __forceinline__ void rank_dense_cells(
     const int pmin = L.min_id;
     const int width = (L.max_id - pmin) / kDenseBuckets + 1;
     for (int t = lo + tid; t < hi; t += nt)
-      atomicAdd(&L.cnt[(ppid[t] - pmin) / width], 1);
+      atomicAdd(&L.cur[(ppid[t] - pmin) / width], 1);
     __syncthreads();
     if (tid == 0) {
       int run = 0;
       for (int b = 0; b < kDenseBuckets; ++b) {
-        L.start[b] = run;
-        run += L.cnt[b];
+        const int c = L.cur[b];
+        L.cur[b] = run;
+        run += c;
       }
     }
     __syncthreads();
     for (int t = lo + tid; t < hi; t += nt) {
       const int p = ppid[t];
       const int b = (p - pmin) / width;
-      scratch[lo + L.start[b] + atomicAdd(&L.cur[b], 1)] = make_int2(p, t);
+      scratch[lo + atomicAdd(&L.cur[b], 1)] = make_int2(p, t);
     }
     __threadfence_block();
     __syncthreads();
     for (int q = lo + tid; q < hi; q += nt) {
       const int2 e = scratch[q];
       const int b = (e.x - pmin) / width;
-      const int b0 = lo + L.start[b], b1 = b0 + L.cnt[b];
-      int r = L.start[b];
+      const int s0 = b > 0 ? L.cur[b - 1] : 0;
+      const int b0 = lo + s0, b1 = lo + L.cur[b];
+      int r = s0;
       for (int u = b0; u < b1; ++u) r += scratch[u].x < e.x ? 1 : 0;
       const int f = lo + r;
       const int src = perm[e.y];
