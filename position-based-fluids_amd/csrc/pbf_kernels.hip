@@ -999,9 +999,9 @@ __global__ __launch_bounds__(256) void k_reorder(
 
 struct HitRecorder {
   unsigned w0, w1, w2, w3;
-  int c;     // entries recorded
-  int g;     // full groups stored
-  int next;  // value of c at which the group in the registers is full
+  unsigned room;  // entries the group in the registers takes after the
+                  // next one (kPerGroup - 1 .. 0)
+  int g;          // full groups stored
   __device__ __forceinline__ void push(unsigned e) {
     w0 = __builtin_amdgcn_alignbit(w1, w0, kEntryBits);
     w1 = __builtin_amdgcn_alignbit(w2, w1, kEntryBits);
@@ -1011,16 +1011,17 @@ struct HitRecorder {
   // Record an entry (its low kEntryBits bits).  No branch on the capacity:
   // past kMaxHits the groups overwrite the last one, which the overflow
   // mark (count kMaxHits + 1) makes unused.
+  // The group is full when room, counted down once per entry, borrows: one
+  // subtract whose borrow is the branch condition.
   __device__ __forceinline__ void add(unsigned entry, uint4 *__restrict__ nl,
                                      int stride, int i) {
     push(entry);
-    if (c == next) {
+    if (__builtin_sub_overflow(room, 1u, &room)) {
       nl[(size_t)min(g, kMaxGroups - 1) * stride + i] =
           make_uint4(w0, w1, w2, w3);
       ++g;
-      next += kPerGroup;
+      room = kPerGroup - 1;
     }
-    ++c;
   }
   // Store the last partial group and the count.  The group is padded with
   // entries naming slot i itself (stencil row 4 is the particle's own row;
@@ -1031,7 +1032,8 @@ struct HitRecorder {
   __device__ __forceinline__ void flush(uint4 *__restrict__ nl,
                                        int *__restrict__ ncnt, int stride,
                                        int i, int self_off, bool wide) {
-    const int part = c - g * kPerGroup;  // entries of the partial group
+    const int part = kPerGroup - 1 - (int)room;  // entries of the partial group
+    const int c = g * kPerGroup + part;
     const bool over =
         wide || (part > 0 && (unsigned)self_off >= (1u << kOffBits));
     if (over || c > kMaxHits) {
@@ -1395,7 +1397,7 @@ __global__ __launch_bounds__(256) void k_lambda_paper(
     const int i = lb * blockDim.x + threadIdx.x;
     if (i - sc.slot_off >= sc.n) continue;
     const float3 xi = packed_at(XP, i);
-    HitRecorder rec{0u, 0u, 0u, 0u, 0, 0, kPerGroup - 1};
+    HitRecorder rec{0u, 0u, 0u, 0u, kPerGroup - 1, 0};
     LambdaSums ls;
     const float h = sc.h, h2 = sc.h2;
     const Gather<Packed3> xpg(XP, stride);
