@@ -325,6 +325,7 @@ pbf::StepShape make_shape(const pbf_params &p0, const pbf::Scene *scenes,
   sh.vorticity = (sh.physics == pbf::kPaper && vort) ? 1 : 0;
   sh.scorr_n4 = 1;
   for (size_t s = 0; s < nscenes; ++s) sh.scorr_n4 &= scenes[s].scorr_n == 4;
+  sh.ref_rn = slots >= pbf::kRefRnSlots ? 1 : 0;
   for (int c = 0; c < pbf::kNumClasses; ++c) {
     sh.block[c] = p0.block_size ? p0.block_size : 128;
     sh.grid[c] = p0.launch_grid;

@@ -441,6 +441,7 @@ __device__ __forceinline__ float dist2_exact(float3 xi, float4 xj, float &rx,
 }
 
 // compute_scaling.cl:56-84 for one list entry.
+template <bool RN>
 __device__ __forceinline__ bool ref_lambda_pair(const Scene &sc, float3 xi,
                                                 float4 xj, float &dens,
                                                 float &gsum, float &gix,
@@ -451,13 +452,14 @@ __device__ __forceinline__ bool ref_lambda_pair(const Scene &sc, float3 xi,
   const bool hit = r2 > 0.0f && r2 < sc.h2;
   if (hit) {
     float ux, uy, uz;
-    const float rl = ref_unit(rx, ry, rz, r2, sc.h2 <= 0x1p40f, ux, uy, uz);
+    const float rl =
+        ref_unit<RN>(rx, ry, rz, r2, sc.h2 <= 0x1p40f, ux, uy, uz);
     const float gx = ux * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float gy = uy * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float gz = uz * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float w = sc.poly6 * (sc.h2 - r2) * (sc.h2 - r2) * (sc.h2 - r2);
     dens += w;
-    gsum += ref_sqrt(gx * gx + gy * gy + gz * gz);
+    gsum += ref_sqrt<RN>(gx * gx + gy * gy + gz * gz);
     gix += gx;
     giy += gy;
     giz += gz;
@@ -476,6 +478,7 @@ __device__ __forceinline__ float ref_lambda_final(const Scene &sc, float dens,
 }
 
 // compute_delta.cl:56-80 for one list entry.
+template <bool RN>
 __device__ __forceinline__ void ref_delta_pair(const Scene &sc, float3 xi,
                                                float4 xj, float li, float lj,
                                                float &sx, float &sy,
@@ -486,7 +489,8 @@ __device__ __forceinline__ void ref_delta_pair(const Scene &sc, float3 xi,
   if (r2 > 0.0f && r2 < sc.h2) {
     // -1 * rx / rl: -(rx / rl) (negation is exact)
     float ux, uy, uz;
-    const float rl = ref_unit(rx, ry, rz, r2, sc.h2 <= 0x1p40f, ux, uy, uz);
+    const float rl =
+        ref_unit<RN>(rx, ry, rz, r2, sc.h2 <= 0x1p40f, ux, uy, uz);
     const float gx = -ux * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float gy = -uy * sc.spiky * (sc.h - rl) * (sc.h - rl);
     const float gz = -uz * sc.spiky * (sc.h - rl) * (sc.h - rl);
@@ -540,6 +544,7 @@ __device__ __forceinline__ float3 clamp_update(const Scene &sc, float3 xi,
 }
 
 // apply_vorticity_and_viscosity.cl:56-61 for one list entry.
+template <bool RN>
 __device__ __forceinline__ void ref_xsph_pair(const Scene &sc, float3 xi,
                                               float4 xj, float4 vi, float4 vj,
                                               float &sx, float &sy,
@@ -549,7 +554,7 @@ __device__ __forceinline__ void ref_xsph_pair(const Scene &sc, float3 xi,
   const float r2 = dist2_exact(xi, xj, rx, ry, rz);
   if (r2 > 0.0f && r2 < sc.h2) {
     const float w = sc.poly6 * (sc.h2 - r2) * (sc.h2 - r2) * (sc.h2 - r2);
-    const float inv = ref_recip(xj.w);
+    const float inv = ref_recip<RN>(xj.w);
     sx += inv * (vj.x - vi.x) * w;
     sy += inv * (vj.y - vi.y) * w;
     sz += inv * (vj.z - vi.z) * w;
@@ -1206,7 +1211,7 @@ __device__ __forceinline__ void for_stencil_ref_queued(
 }
 
 // compute_scaling.cl:12-140.
-template <bool PT>
+template <bool PT, bool RN>
 __global__ __launch_bounds__(256) void k_lambda_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
@@ -1221,7 +1226,7 @@ __global__ __launch_bounds__(256) void k_lambda_ref(
     const float3 xi = make_float3(x4.x, x4.y, x4.z);
     float dens = 0.0f, gsum = 0.0f, gix = 0.0f, giy = 0.0f, giz = 0.0f;
     for_stencil_ref_queued(sc, cs, X, xi, i, rq_lds, [&](int j) {
-      ref_lambda_pair(sc, xi, X[j], dens, gsum, gix, giy, giz);
+      ref_lambda_pair<RN>(sc, xi, X[j], dens, gsum, gix, giy, giz);
     });
     lam[i] = ref_lambda_final(sc, dens, gsum, gix, giy, giz);
     rho[i] = dens;
@@ -1231,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_lambda_ref(
 // compute_delta.cl:13-152 + update_predicted.cl.  LAST also forms
 // v = (x* - x)/dt (update_velocities.cl:8) and carries rho of the last
 // lambda pass in .w for the XSPH pass.
-template <bool PT, bool LAST>
+template <bool PT, bool LAST, bool RN>
 __global__ __launch_bounds__(256) void k_delta_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
@@ -1248,7 +1253,7 @@ __global__ __launch_bounds__(256) void k_delta_ref(
     const float li = lam[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      if (j != i) ref_delta_pair(sc, xi, X[j], li, lam[j], sx, sy, sz);
+      if (j != i) ref_delta_pair<RN>(sc, xi, X[j], li, lam[j], sx, sy, sz);
     });
     const float3 dp = make_float3(sx / sc.rho0, sy / sc.rho0, sz / sc.rho0);
     const float3 xn = clamp_update(sc, xi, dp, false);
@@ -1289,7 +1294,7 @@ __device__ __forceinline__ void write_final(int i, float3 xs, float4 v,
 }
 
 // apply_vorticity_and_viscosity.cl:12-104 fused with update_positions.cl.
-template <bool PT>
+template <bool PT, bool RN>
 __global__ __launch_bounds__(256) void k_xsph_final_ref(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float4 *__restrict__ X, const int *__restrict__ cs,
@@ -1306,7 +1311,7 @@ __global__ __launch_bounds__(256) void k_xsph_final_ref(
     const float4 vi = V[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for_stencil_ref(sc, cs, xi.x, xi.y, xi.z, [&](int j) {
-      if (j != i) ref_xsph_pair(sc, xi, X[j], vi, V[j], sx, sy, sz);
+      if (j != i) ref_xsph_pair<RN>(sc, xi, X[j], vi, V[j], sx, sy, sz);
     });
     const float4 dv = mul_exact(sc.c_xsph, sx, sy, sz);
     float len;
@@ -2044,20 +2049,30 @@ static NextSort next_sort(const DevBuffers &d) {
   return NextSort{d.pred_u, d.key, d.rank, d.count};
 }
 
-// A REF delta pass.
+// A REF delta pass (RN: the shape's choice of IEEE sequences, ref_unit).
 template <bool LAST>
 static void delta_ref_pass(const DevBuffers &d, Launcher &L,
                            const float4 *cur, float4 *nxt) {
-  PBF_GATHER(L, kClsDelta, (k_delta_ref<true, LAST>),
-             (k_delta_ref<false, LAST>), 0, d.scenes, d.block_scene, cur,
-             d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
+  if (L.sh.ref_rn)
+    PBF_GATHER(L, kClsDelta, (k_delta_ref<true, LAST, true>),
+               (k_delta_ref<false, LAST, true>), 0, d.scenes, d.block_scene,
+               cur, d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
+  else
+    PBF_GATHER(L, kClsDelta, (k_delta_ref<true, LAST, false>),
+               (k_delta_ref<false, LAST, false>), 0, d.scenes, d.block_scene,
+               cur, d.cell_start, d.lam, d.rho, d.xold, nxt, d.v);
 }
 
 static void lambda_ref_pass(const DevBuffers &d, Launcher &L,
                             const float4 *cur) {
-  PBF_GATHER(L, kClsLambda, (k_lambda_ref<true>), (k_lambda_ref<false>),
-             kRefQueue, d.scenes, d.block_scene, cur, d.cell_start, d.lam,
-             d.rho);
+  if (L.sh.ref_rn)
+    PBF_GATHER(L, kClsLambda, (k_lambda_ref<true, true>),
+               (k_lambda_ref<false, true>), kRefQueue, d.scenes,
+               d.block_scene, cur, d.cell_start, d.lam, d.rho);
+  else
+    PBF_GATHER(L, kClsLambda, (k_lambda_ref<true, false>),
+               (k_lambda_ref<false, false>), kRefQueue, d.scenes,
+               d.block_scene, cur, d.cell_start, d.lam, d.rho);
 }
 
 static void step_ref(const DevBuffers &d, Launcher &L) {
@@ -2072,9 +2087,12 @@ static void step_ref(const DevBuffers &d, Launcher &L) {
       delta_ref_pass<false>(d, L, cur, nxt);
     std::swap(cur, nxt);
   }
-  PBF_GATHER(L, kClsXsph, (k_xsph_final_ref<true>), (k_xsph_final_ref<false>),
-             0, d.scenes, d.block_scene, cur, d.cell_start, d.v, d.pid_s,
-             d.pos, d.vel, d.pid, d.err, next_sort(d));
+  // XSPH's one reciprocal per pair: the range-tested sequence at every
+  // size (c0: 121 against 125 us per launch with the general one)
+  PBF_GATHER(L, kClsXsph, (k_xsph_final_ref<true, true>),
+             (k_xsph_final_ref<false, true>), 0, d.scenes, d.block_scene, cur,
+             d.cell_start, d.v, d.pid_s, d.pos, d.vel, d.pid, d.err,
+             next_sort(d));
 }
 
 // A delta pass, at the s_corr exponent instance of the step's scenes.
@@ -2347,7 +2365,7 @@ void enqueue_probe(const DevBuffers &d, const StepShape &sh, hipStream_t st) {
     sort_stage<kRef>(d, L, 1, true);
     k_count_nbrs<kRef><<<grid, bs, 0, st>>>(d.scenes, d.block_scene, d.xa,
                                             d.cell_start, d.nbr);
-    k_lambda_ref<false><<<grid, bs, kRefQueue * bs * sizeof(int), st>>>(
+    k_lambda_ref<false, false><<<grid, bs, kRefQueue * bs * sizeof(int), st>>>(
         d.scenes, d.block_scene, d.xa, d.cell_start, d.lam, d.rho);
   } else {
     sort_stage<kPaper>(d, L, 1, true);

@@ -40,6 +40,10 @@ constexpr int kDenseBuckets = 1024;
 // per cell: the 32,768-particle dam break in PAPER physics uses 14 MB).
 constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
 
+// Slots from which REF's pair terms use the range-tested IEEE sequences
+// (the same bits as the general ones).
+constexpr int kRefRnSlots = 1 << 18;
+
 // StepShape::grid value of a persistent launch sized to the device's
 // resident work-groups.
 constexpr int kGridResident = -1;
@@ -71,6 +75,10 @@ struct StepShape {
   int fp16_iterations;  // leading iterations gathering FP16 positions
   int vorticity;    // PAPER: vorticity confinement pass enabled
   int scorr_n4;     // every scene has the s_corr exponent n = 4
+  int ref_rn;       // REF lambda and delta: range-tested shortened IEEE
+                    // sequences (from kRefRnSlots slots: the step is
+                    // throughput-bound) or the general ones (small steps,
+                    // latency-bound)
   int block[kNumClasses];  // work-group size (64, 128 or 256) per class
   // launch form per class: 0 = one work-group per tile of block[c] slots;
   // g > 0 = persistent, g work-groups (a multiple of 8) walking the tiles;

@@ -123,12 +123,16 @@ __device__ __forceinline__ bool pair_rn_ok(float rx, float ry, float rz,
 
 // |r| and r / |r| of a pair with r^2 < h^2 (r^2 formed by the caller),
 // each correctly rounded: sqrt(r^2), then one division per component.
-// h2_ok: the scene's h^2 <= 2^40.
+// h2_ok: the scene's h^2 <= 2^40.  RN: the shortened sequences where the
+// pair's operands allow (fewer instructions per pair, a range test per
+// pair); without RN the general ones only (the same bits; a shorter
+// dependent chain for the scenes whose time a wave's latency sets).
+template <bool RN>
 __device__ __forceinline__ float ref_unit(float rx, float ry, float rz,
                                          float r2, bool h2_ok, float &ux,
                                          float &uy, float &uz) {
   float rl;
-  if (pair_rn_ok(rx, ry, rz, r2, h2_ok)) {
+  if (RN && pair_rn_ok(rx, ry, rz, r2, h2_ok)) {
     rl = sqrt_rn(r2);
     const RcpRN d = rcp_refined(rl);
     ux = div_rn(rx, d);
@@ -143,14 +147,16 @@ __device__ __forceinline__ float ref_unit(float rx, float ry, float rz,
   return rl;
 }
 
-// sqrt(x) correctly rounded.
+// sqrt(x) correctly rounded (RN as for ref_unit).
+template <bool RN>
 __device__ __forceinline__ float ref_sqrt(float x) {
-  return sqrt_ok(x) ? sqrt_rn(x) : sqrtf(x);
+  return RN && sqrt_ok(x) ? sqrt_rn(x) : sqrtf(x);
 }
 
-// 1 / b correctly rounded.
+// 1 / b correctly rounded (RN as for ref_unit).
+template <bool RN>
 __device__ __forceinline__ float ref_recip(float b) {
-  return recip_ok(b) ? recip_rn(b) : 1.0f / b;
+  return RN && recip_ok(b) ? recip_rn(b) : 1.0f / b;
 }
 
 }  // namespace pbf

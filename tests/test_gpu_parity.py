@@ -128,6 +128,23 @@ def test_ref_trajectory_c0(c0_ref, nsteps):
     assert np.array_equal(gv, ov)      # v, and the mass in .w
 
 
+def test_ref_range_tested_sequences_at_large_size():
+    """A REF step of 2^18 slots or more takes the pair terms' range-tested
+    IEEE sequences, a smaller one the general sequences (the REF tests
+    above); both give the FP32 oracle's bits: a 262,144-particle block, two
+    steps."""
+    say("ref 64^3 block, range-tested sequences")
+    sc = scenarios.small_dam(dims=(64, 64, 64), box=(4.0, 4.0, 4.0),
+                             physics=pbf.PHYSICS_REF, seed=17)
+    p, parts = sc.params[0], sc.particles[0]
+    assert len(parts) >= 1 << 18
+    gp, gv = run_gpu(p, parts, 2)
+    pos, vel = orc.state_from_particles(parts)
+    op, ov = orc.run_f32(p, pos, vel, 2)
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gv, ov)
+
+
 def test_ref_two_particles(golden_dir):
     say("ref two.in")
     p, _ = pbf.read_par(os.path.join(golden_dir, "two.par"))
