@@ -56,7 +56,7 @@ __global__ void k_check(int which, const float *__restrict__ in, long n,
       const float rx = in[3 * i], ry = in[3 * i + 1], rz = in[3 * i + 2];
       const float r2 = rx * rx + ry * ry + rz * rz;
       float ux, uy, uz;
-      const float rl = pbf::ref_unit(rx, ry, rz, r2, true, ux, uy, uz);
+      const float rl = pbf::ref_unit<true>(rx, ry, rz, r2, true, ux, uy, uz);
       const float rl0 = sqrtf(r2);
       note(same(rl, rl0) && same(ux, rx / rl0) && same(uy, ry / rl0) &&
                same(uz, rz / rl0),
@@ -65,12 +65,12 @@ __global__ void k_check(int which, const float *__restrict__ in, long n,
     }
     case 4: {
       const float x = in[i];
-      note(same(pbf::ref_sqrt(x), sqrtf(x)), i, bad, first);
+      note(same(pbf::ref_sqrt<true>(x), sqrtf(x)), i, bad, first);
       break;
     }
     case 5: {
       const float b = in[i];
-      note(same(pbf::ref_recip(b), 1.0f / b), i, bad, first);
+      note(same(pbf::ref_recip<true>(b), 1.0f / b), i, bad, first);
       break;
     }
     case 6:
