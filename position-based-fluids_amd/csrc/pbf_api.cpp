@@ -428,10 +428,10 @@ void build(pbf_sim &S, uint32_t nscenes, const pbf_params *params,
   }
   S.total = (int)ids;
   for (pbf::Scene &sc : S.scenes) sc.slots_all = (int)slot;
-  // PAPER gathers address arrays of up to 32-byte records with 32-bit byte
+  // PAPER gathers address arrays of up to 16-byte records with 32-bit byte
   // offsets
-  if (params[0].physics == PBF_PHYSICS_PAPER && slot > INT32_MAX / 32)
-    invalid("problem too large for PAPER physics (more than 2^26 slots)");
+  if (params[0].physics == PBF_PHYSICS_PAPER && slot > INT32_MAX / 16)
+    invalid("problem too large for PAPER physics (more than 2^27 slots)");
   const pbf_params &p0 = params[0];
   pbf::StepShape &sh = S.shape;
   sh = make_shape(p0, S.scenes.data(), S.scenes.size(), params, slot, cells);

@@ -214,18 +214,11 @@ struct Gather {
       const U3 v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
       return Pos3{__uint_as_float(v.x), __uint_as_float(v.y),
                   __uint_as_float(v.z)};
-    } else if constexpr (sizeof(T) == 32) {
-      T t;
-      t.x = __builtin_bit_cast(
-          float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-      t.v = __builtin_bit_cast(
-          float4, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0));
-      return t;
     } else if constexpr (sizeof(T) == 16) {
       return __builtin_bit_cast(
           T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
     } else {
-      static_assert(sizeof(T) == 8, "32-, 16- or 8-byte records");
+      static_assert(sizeof(T) == 8, "16- or 8-byte records");
       return __builtin_bit_cast(
           T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
@@ -264,8 +257,8 @@ struct Gather {
 
 // XSPH's neighbour record from two arrays: x from the packed positions (12
 // bytes per slot) and (v, 1/rho) (16 bytes per slot), two loads that touch
-// fewer cache lines than one 32-byte record's two halves.  Offsets are in
-// units of 4 bytes per slot.
+// fewer cache lines than the two halves of one 32-byte record would.
+// Offsets are in units of 4 bytes per slot.
 struct PosVel {};
 
 template <>
