@@ -1677,7 +1677,7 @@ constexpr float kOmegaScale = 1.0f / 256.0f;
 // N_i; writes B = (x, |omega|) for the vorticity pass, or (QIN) its 16-bit
 // records QA.
 template <bool PT, bool QIN>
-__global__ __launch_bounds__(256) void k_xsph_omega(
+__device__ __forceinline__ void xsph_omega_body(
     const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
     const float *__restrict__ XS, const float *__restrict__ XC,
     const float4 *__restrict__ VR, const float4 *__restrict__ xold,
@@ -1712,6 +1712,35 @@ __global__ __launch_bounds__(256) void k_xsph_omega(
     else
       B[i] = make_float4(xi.x, xi.y, xi.z, wl);
   }
+}
+
+// The FP32 instance at the compiler's register count (4 waves per SIMD; at 5
+// it spills); the 16-bit instance capped at 5 waves, which it fits without
+// spilling (c4 mixed: XSPH + omega 283.5 against 294.9 us).
+template <bool PT>
+__global__ __launch_bounds__(256) void k_xsph_omega(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float *__restrict__ XS, const float *__restrict__ XC,
+    const float4 *__restrict__ VR, const float4 *__restrict__ xold,
+    const uint4 *__restrict__ QV, const int *__restrict__ cs,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride, float4 *__restrict__ VX,
+    float4 *__restrict__ OM, float4 *__restrict__ B,
+    uint2 *__restrict__ QA) {
+  xsph_omega_body<PT, false>(scenes, block_scene, XS, XC, VR, xold, QV, cs, nl, ncnt, rbase, stride, VX, OM, B, QA);
+}
+template <bool PT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void
+k_xsph_omega_q(
+    const Scene *__restrict__ scenes, const int *__restrict__ block_scene,
+    const float *__restrict__ XS, const float *__restrict__ XC,
+    const float4 *__restrict__ VR, const float4 *__restrict__ xold,
+    const uint4 *__restrict__ QV, const int *__restrict__ cs,
+    const uint4 *__restrict__ nl, const int *__restrict__ ncnt,
+    const int *__restrict__ rbase, int stride, float4 *__restrict__ VX,
+    float4 *__restrict__ OM, float4 *__restrict__ B,
+    uint2 *__restrict__ QA) {
+  xsph_omega_body<PT, true>(scenes, block_scene, XS, XC, VR, xold, QV, cs, nl, ncnt, rbase, stride, VX, OM, B, QA);
 }
 
 // eta = sum (|w_j| - |w_i|) grad_i W_ij over N_i, N = eta/|eta|,
@@ -2133,9 +2162,16 @@ static void lambda_list_pass(const DevBuffers &d, Launcher &L) {
 template <bool Q>
 static void vorticity_passes(const DevBuffers &d, Launcher &L) {
   const int P = L.sh.slots;
-  PBF_GATHER(L, kClsXsph, (k_xsph_omega<true, Q>), (k_xsph_omega<false, Q>),
-             9, d.scenes, d.block_scene, d.xp, d.xc, d.vr, d.xold, d.qv,
-             d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx, d.om, d.xb, d.qa);
+  if constexpr (Q)
+    PBF_GATHER(L, kClsXsph, (k_xsph_omega_q<true>), (k_xsph_omega_q<false>),
+               9, d.scenes, d.block_scene, d.xp, d.xc, d.vr, d.xold, d.qv,
+               d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx, d.om, d.xb,
+               d.qa);
+  else
+    PBF_GATHER(L, kClsXsph, (k_xsph_omega<true>), (k_xsph_omega<false>), 9,
+               d.scenes, d.block_scene, d.xp, d.xc, d.vr, d.xold, d.qv,
+               d.cell_start, d.nl, d.ncnt, d.rbase, P, d.vx, d.om, d.xb,
+               d.qa);
   PBF_GATHER(L, kClsVort, (k_vort_final<true, Q>), (k_vort_final<false, Q>),
              9, d.scenes, d.block_scene, d.xp, d.xb, d.xc, d.qa, d.cell_start,
              d.vx, d.om, d.nl, d.ncnt, d.rbase, P, d.pid_s, d.pos, d.vel,
